@@ -1,0 +1,145 @@
+/*
+ * tessera_qa: MI355X (gfx950) implementation of TESSERA's quantum-annealing hot
+ * path behind the upstream C ABI.
+ *
+ *   H(s) = -A(s) sum_i X_i - B(s) ( sum_<ij> J_ij Z_i Z_j + sum_i h_i Z_i )
+ *
+ * Two engines:
+ *   - MPS-TEBD (upstream cuda/src/mps.cpp): second-order Trotter, single-site
+ *     half steps around a block of ZZ gates routed by swap networks, every
+ *     two-site gate followed by an SVD truncated to chi.
+ *   - exact state vector (upstream rust/src/quantum.rs): phase(B dt/2),
+ *     X-rotations(A dt), phase(B dt/2) on all 2^n amplitudes.
+ *
+ * Conventions shared with upstream:
+ *   - spins are int8 in {-1,+1}; MPS physical index 0 <-> +1; state-vector bit
+ *     set <-> +1;
+ *   - energies sum edges in list order, then fields (rust/src/ising.rs:23-32);
+ *   - schedules are caller-sampled arrays a[k], b[k], k < steps;
+ *   - all buffers are caller-owned host memory, contiguous, and outlive the
+ *     call; the library keeps no pointer after returning.
+ *
+ * Every entry point is synchronous and may be called from any host thread.
+ * Device work runs on the first visible GPU.
+ */
+#ifndef TESSERA_QA_H
+#define TESSERA_QA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes of the extended entry points. */
+enum {
+    TESSERA_OK = 0,
+    TESSERA_ERR_ARG = -1,        /* null pointer, n == 0, steps < 2, bad edge index */
+    TESSERA_ERR_TOO_LARGE = -2,  /* state does not fit the device */
+    TESSERA_ERR_DEVICE = -3,     /* HIP runtime or kernel launch failure */
+    TESSERA_ERR_NONFINITE = -4   /* NaN or Inf in h, J, schedule or dt */
+};
+
+/*
+ * Drop-in for upstream cuda/src/ffi.cpp:10-16 (bound by rust/src/ffi.rs:19-35):
+ * MPS-TEBD with dt = 0.1 and one ZZ slice per step (ffi.cpp:28-29). Writes
+ * out_s[n] (always) and out_entropy[steps] (qubit-0 entropy after each step,
+ * if non-null); returns the Ising energy of the readout spins
+ * sign(<Z_i>), 0 -> +1. Edge indices are narrowed to int as upstream does.
+ * Never throws; on failure writes out_s = +1 and returns NaN (upstream has no
+ * error path; tessera_last_error() tells what failed).
+ */
+double tessera_qa_anneal(const double* h, size_t n,
+                         const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                         const double* schedule_a, const double* schedule_b, size_t steps,
+                         size_t chi, int8_t* out_s, double* out_entropy);
+
+/*
+ * Extended MPS-TEBD entry point (superset of tessera_qa_anneal; dt = 0.1,
+ * zz_substeps = 1, bias = NULL reproduces it). Replaces the upstream
+ * mps_anneal(...) signature (cuda/include/mps.hpp:76-82) and exposes the
+ * AnnealResult fields the plain ABI drops (mps.hpp:37-44).
+ *   dt            Trotter step
+ *   zz_substeps   ZZ block slices, alternating forward/reverse (mps.cpp:330-341)
+ *   bias          optional per-spin longitudinal warm-start field: the
+ *                 single-site generator becomes -A X - (B h_i + A bias_i) Z,
+ *                 which vanishes with A at the end of the anneal (NULL = none)
+ *   out_scalars   [4] = energy, expected_energy <psi|H|psi>, prob of the readout,
+ *                 max_trunc (largest discarded weight over the run)
+ *   out_entropy   [steps] qubit-0 entropy trace (nullable)
+ *   out_max_bond  [steps] largest bond after each step (nullable)
+ * Returns TESSERA_OK or a negative status.
+ */
+int tessera_qa_anneal_ex(const double* h, size_t n,
+                         const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                         const double* schedule_a, const double* schedule_b, size_t steps,
+                         double dt, size_t chi, int zz_substeps, const double* bias,
+                         int8_t* out_s, double* out_scalars, double* out_entropy,
+                         int32_t* out_max_bond);
+
+/*
+ * Exact state-vector anneal with an arbitrary schedule: the C form of
+ * rust/src/quantum.rs:63 anneal_statevector (QaOutcome fields, quantum.rs:27-43).
+ * No n <= 28 cap: n is limited by device memory (16 * 2^n bytes).
+ *   out_s[n]                 argmax configuration (first index on ties)
+ *   out_prob                 success probability of out_s            (nullable)
+ *   out_entropy[steps]       qubit-0 entropy trace                    (nullable)
+ *   out_energy_trace[steps]  <H_problem> after each step              (nullable)
+ *   out_energy               Ising energy of out_s                    (nullable)
+ *   out_expected_energy      final <H_problem>                        (nullable)
+ */
+int tessera_sv_anneal(const double* h, size_t n,
+                      const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                      const double* schedule_a, const double* schedule_b, size_t steps, double dt,
+                      int8_t* out_s, double* out_prob, double* out_entropy, double* out_energy_trace,
+                      double* out_energy, double* out_expected_energy);
+
+/*
+ * Drop-in for upstream rust/src/capi.rs:107-143 (the ctypes path of
+ * python/test_ffi.py and python/tessera_cam.py): state-vector anneal on the
+ * smooth cos^2/sin^2 schedule with steps = max(steps, 2). Returns the energy
+ * of the readout; 0.0 when h or out_s is null; NaN when n > 24, as upstream.
+ */
+double tessera_solve_qa(const double* h, size_t n,
+                        const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                        size_t steps, double dt, int8_t* out_s, double* out_prob,
+                        double* out_entropy, double* out_energy);
+
+/* ---- native TEBD session (no upstream counterpart: the runtime the entry
+ * points above are built on, exposed for benchmarking and batched drivers) ---- */
+typedef struct tessera_tebd tessera_tebd;
+
+/* Allocate device state for n sites at bond dimension chi; model as above. */
+int tessera_tebd_create(const double* h, size_t n, const size_t* ei, const size_t* ej,
+                        const double* ew, size_t m, size_t chi, double dt, int zz_substeps,
+                        tessera_tebd** out);
+/* |+>^n product state (mps.cpp:9-17). */
+int tessera_tebd_reset(tessera_tebd* t);
+/* Random state with every bond at its capacity min(chi, 2^b, 2^(n-b)) and
+ * bond weights exp(-k/decay); the steady state of a saturated run. */
+int tessera_tebd_randomize(tessera_tebd* t, uint64_t seed, double decay);
+/* Enqueue `count` Trotter steps with schedule values a[k], b[k] (no host sync);
+ * qubit-0 entropy of each step is recorded on the device when record != 0. */
+int tessera_tebd_run(tessera_tebd* t, const double* a, const double* b, size_t count, int record);
+/* Wait for queued work; copy the recorded entropy trace (nullable), the max
+ * discarded weight and the current bond dimensions (n+1 values, nullable). */
+int tessera_tebd_sync(tessera_tebd* t, double* out_entropy, size_t cap, double* out_max_trunc,
+                      int32_t* out_bonds);
+/* Readout: spins, energy, expected energy, prob (as tessera_qa_anneal_ex). */
+int tessera_tebd_readout(tessera_tebd* t, int8_t* out_s, double* out_scalars);
+/* Dense amplitudes (n <= 24): out[2*x], out[2*x+1] = re, im of psi(x), where bit
+ * i of x is the physical index of site i (0 <-> spin +1). */
+int tessera_tebd_dense(tessera_tebd* t, double* out);
+/* Number of two-site updates one Trotter step performs for this model. */
+size_t tessera_tebd_updates_per_step(const tessera_tebd* t);
+void tessera_tebd_destroy(tessera_tebd* t);
+
+/* Text of the last error on this thread ("" if none). */
+const char* tessera_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TESSERA_QA_H */

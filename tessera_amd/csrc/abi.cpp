@@ -1,0 +1,311 @@
+// extern "C" entry points of libtessera_qa.so (declared in include/tessera_qa.h).
+// Every entry point catches all exceptions: nothing unwinds across the C ABI.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/tessera_qa.h"
+#include "common.hpp"
+#include "sv.hpp"
+#include "tebd.hpp"
+
+namespace tq {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+namespace {
+
+bool finite_all(const double* p, std::size_t n) {
+    for (std::size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+// Shared validation of the model arguments; throws tq::Error.
+void check_model(const double* h, std::size_t n, const std::size_t* ei, const std::size_t* ej, const double* ew,
+                 std::size_t m, const double* a, const double* b, std::size_t steps) {
+    if (!h || n == 0) throw Error(TESSERA_ERR_ARG, "h is null or n == 0");
+    if (m > 0 && (!ei || !ej || !ew)) throw Error(TESSERA_ERR_ARG, "edge arrays are null with m > 0");
+    if (steps > 0 && (!a || !b)) throw Error(TESSERA_ERR_ARG, "schedule arrays are null");
+    for (std::size_t k = 0; k < m; ++k)
+        if (ei[k] >= n || ej[k] >= n) throw Error(TESSERA_ERR_ARG, "edge index >= n");
+    if (!finite_all(h, n) || (m && !finite_all(ew, m)) || (steps && (!finite_all(a, steps) || !finite_all(b, steps))))
+        throw Error(TESSERA_ERR_NONFINITE, "non-finite value in h, J or schedule");
+}
+
+TebdModel make_model(const double* h, std::size_t n, const std::size_t* ei, const std::size_t* ej, const double* ew,
+                     std::size_t m, const double* bias) {
+    TebdModel md;
+    md.n = (int)n;
+    md.h.assign(h, h + n);
+    md.ei.resize(m);
+    md.ej.resize(m);
+    for (std::size_t k = 0; k < m; ++k) {  // narrowed to int as ffi.cpp:19-21
+        md.ei[k] = (int)ei[k];
+        md.ej[k] = (int)ej[k];
+    }
+    md.ew.assign(ew, ew + m);
+    if (bias) md.bias.assign(bias, bias + n);
+    return md;
+}
+
+int status_of(const std::exception& e) {
+    if (auto* te = dynamic_cast<const Error*>(&e)) return te->status;
+    return TESSERA_ERR_DEVICE;
+}
+
+}  // namespace
+}  // namespace tq
+
+struct tessera_tebd {
+    std::unique_ptr<tq::Tebd> eng;
+    int recorded = 0;
+};
+
+using namespace tq;
+
+extern "C" {
+
+const char* tessera_last_error(void) { return tq::last_error(); }
+
+int tessera_qa_anneal_ex(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                         const double* schedule_a, const double* schedule_b, size_t steps, double dt, size_t chi,
+                         int zz_substeps, const double* bias, int8_t* out_s, double* out_scalars,
+                         double* out_entropy, int32_t* out_max_bond) {
+    try {
+        set_last_error("");
+        check_model(h, n, ei, ej, ew, m, schedule_a, schedule_b, steps);
+        if (!out_s || !out_scalars) throw Error(TESSERA_ERR_ARG, "out_s or out_scalars is null");
+        if (!std::isfinite(dt)) throw Error(TESSERA_ERR_NONFINITE, "dt is not finite");
+        if (bias && !finite_all(bias, n)) throw Error(TESSERA_ERR_NONFINITE, "non-finite bias");
+        if (chi == 0) throw Error(TESSERA_ERR_ARG, "chi == 0");
+        Tebd t(make_model(h, n, ei, ej, ew, m, bias), (int)chi, dt, zz_substeps);
+        for (size_t k = 0; k < steps; ++k) {
+            t.step(schedule_a[k], schedule_b[k], (int)k);
+            if (out_max_bond) {
+                const std::vector<int> d = t.bonds();
+                int mb = 1;
+                for (int i = 1; i <= (int)n; ++i) mb = std::max(mb, d[i]);
+                out_max_bond[k] = mb;
+            }
+        }
+        t.sync();
+        if (out_entropy && steps) {
+            const std::vector<double> ent = t.entropy((int)steps);
+            std::memcpy(out_entropy, ent.data(), steps * sizeof(double));
+        }
+        const Tebd::Readout r = t.readout();
+        std::memcpy(out_s, r.spins.data(), n);
+        out_scalars[0] = r.energy;
+        out_scalars[1] = r.expected_energy;
+        out_scalars[2] = r.prob;
+        out_scalars[3] = t.max_trunc();
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    } catch (...) {
+        set_last_error("unknown exception");
+        return TESSERA_ERR_DEVICE;
+    }
+}
+
+double tessera_qa_anneal(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                         const double* schedule_a, const double* schedule_b, size_t steps, size_t chi, int8_t* out_s,
+                         double* out_entropy) {
+    double sc[4] = {0, 0, 0, 0};
+    const int st = tessera_qa_anneal_ex(h, n, ei, ej, ew, m, schedule_a, schedule_b, steps, 0.1, chi, 1, nullptr,
+                                        out_s, sc, out_entropy, nullptr);
+    if (st != TESSERA_OK) {
+        if (out_s)
+            for (size_t i = 0; i < n; ++i) out_s[i] = 1;
+        return std::nan("");
+    }
+    return sc[0];
+}
+
+int tessera_sv_anneal(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                      const double* schedule_a, const double* schedule_b, size_t steps, double dt, int8_t* out_s,
+                      double* out_prob, double* out_entropy, double* out_energy_trace, double* out_energy,
+                      double* out_expected_energy) {
+    try {
+        set_last_error("");
+        check_model(h, n, ei, ej, ew, m, schedule_a, schedule_b, steps);
+        if (!out_s) throw Error(TESSERA_ERR_ARG, "out_s is null");
+        if (steps < 1) throw Error(TESSERA_ERR_ARG, "steps < 1");
+        if (!std::isfinite(dt)) throw Error(TESSERA_ERR_NONFINITE, "dt is not finite");
+        if (n > 36) throw Error(TESSERA_ERR_TOO_LARGE, "n > 36 does not fit one device");
+        SvProblem pb;
+        pb.n = (int)n;
+        pb.h.assign(h, h + n);
+        pb.ei.resize(m);
+        pb.ej.resize(m);
+        for (size_t k = 0; k < m; ++k) {
+            pb.ei[k] = (int)ei[k];
+            pb.ej[k] = (int)ej[k];
+        }
+        pb.ew.assign(ew, ew + m);
+        pb.a.assign(schedule_a, schedule_a + steps);
+        pb.b.assign(schedule_b, schedule_b + steps);
+        pb.steps = (int)steps;
+        pb.dt = dt;
+        const SvOutcome r = sv_anneal_device(pb, false);
+        std::memcpy(out_s, r.spins.data(), n);
+        if (out_prob) *out_prob = r.success_prob;
+        if (out_entropy) std::memcpy(out_entropy, r.entropy.data(), steps * sizeof(double));
+        if (out_energy_trace) std::memcpy(out_energy_trace, r.energy_trace.data(), steps * sizeof(double));
+        if (out_energy) *out_energy = r.energy;
+        if (out_expected_energy) *out_expected_energy = r.expected_energy;
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    } catch (...) {
+        set_last_error("unknown exception");
+        return TESSERA_ERR_DEVICE;
+    }
+}
+
+// rust/src/capi.rs:107-143 semantics on the device state vector.
+double tessera_solve_qa(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                        size_t steps, double dt, int8_t* out_s, double* out_prob, double* out_entropy,
+                        double* out_energy) {
+    if (!h || !out_s) return 0.0;
+    if (n > 24) return std::nan("");
+    const size_t st = steps < 2 ? 2 : steps;
+    std::vector<double> a(st), b(st);
+    for (size_t k = 0; k < st; ++k) {  // schedule.rs:35-46
+        const double s = (double)k / (double)(st - 1);
+        const double th = 1.5707963267948966 * s;
+        const double c = std::cos(th), sn = std::sin(th);
+        a[k] = c * c;
+        b[k] = sn * sn;
+    }
+    // capi.rs:32-39 treats a null ei as "no couplings"
+    const size_t mm = (m > 0 && ei) ? m : 0;
+    double energy = std::nan("");
+    const int rc = tessera_sv_anneal(h, n, mm ? ei : nullptr, mm ? ej : nullptr, mm ? ew : nullptr, mm, a.data(),
+                                     b.data(), st, dt, out_s, out_prob, out_entropy, out_energy, &energy, nullptr);
+    if (rc != TESSERA_OK) return std::nan("");
+    return energy;
+}
+
+int tessera_tebd_create(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                        size_t chi, double dt, int zz_substeps, tessera_tebd** out) {
+    try {
+        set_last_error("");
+        if (!out) throw Error(TESSERA_ERR_ARG, "out is null");
+        check_model(h, n, ei, ej, ew, m, nullptr, nullptr, 0);
+        if (chi == 0) throw Error(TESSERA_ERR_ARG, "chi == 0");
+        auto* t = new tessera_tebd;
+        try {
+            t->eng = std::make_unique<Tebd>(make_model(h, n, ei, ej, ew, m, nullptr), (int)chi, dt, zz_substeps);
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        *out = t;
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_reset(tessera_tebd* t) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        t->eng->reset_product();
+        t->recorded = 0;
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_randomize(tessera_tebd* t, uint64_t seed, double decay) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        if (!(decay > 0)) throw Error(TESSERA_ERR_ARG, "decay must be > 0");
+        t->eng->randomize(seed, decay);
+        t->recorded = 0;
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_run(tessera_tebd* t, const double* a, const double* b, size_t count, int record) {
+    try {
+        if (!t || (count && (!a || !b))) throw Error(TESSERA_ERR_ARG, "null argument");
+        if (record) t->eng->ensure_trace(t->recorded + (int)count);
+        for (size_t k = 0; k < count; ++k) {
+            t->eng->step(a[k], b[k], record ? t->recorded : -1);
+            if (record) ++t->recorded;
+        }
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_sync(tessera_tebd* t, double* out_entropy, size_t cap, double* out_max_trunc, int32_t* out_bonds) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        t->eng->sync();
+        if (out_entropy && cap) {
+            const int cnt = (int)std::min<size_t>(cap, (size_t)t->recorded);
+            const std::vector<double> e = t->eng->entropy(cnt);
+            std::memcpy(out_entropy, e.data(), cnt * sizeof(double));
+        }
+        if (out_max_trunc) *out_max_trunc = t->eng->max_trunc();
+        if (out_bonds) {
+            const std::vector<int> d = t->eng->bonds();
+            for (size_t i = 0; i < d.size(); ++i) out_bonds[i] = d[i];
+        }
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_readout(tessera_tebd* t, int8_t* out_s, double* out_scalars) {
+    try {
+        if (!t || !out_s || !out_scalars) throw Error(TESSERA_ERR_ARG, "null argument");
+        const Tebd::Readout r = t->eng->readout();
+        std::memcpy(out_s, r.spins.data(), r.spins.size());
+        out_scalars[0] = r.energy;
+        out_scalars[1] = r.expected_energy;
+        out_scalars[2] = r.prob;
+        out_scalars[3] = t->eng->max_trunc();
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_dense(tessera_tebd* t, double* out) {
+    try {
+        if (!t || !out) throw Error(TESSERA_ERR_ARG, "null argument");
+        const std::vector<double> d = t->eng->dense();
+        std::memcpy(out, d.data(), d.size() * sizeof(double));
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+size_t tessera_tebd_updates_per_step(const tessera_tebd* t) { return t ? t->eng->updates_per_step() : 0; }
+
+void tessera_tebd_destroy(tessera_tebd* t) { delete t; }
+
+}  // extern "C"

@@ -1,0 +1,821 @@
+// MPS-TEBD engine on gfx950 (upstream cuda/src/mps.cpp, cuda/src/linalg.cpp).
+//
+// HBM layout: site i is a (Dl, 2, Dr) complex128 tensor stored with fixed
+// capacity strides, element (a, s, b) at off[i] + (a*2 + s)*cap[i+1] + b, where
+// cap[b] = min(chi, 2^b, 2^(n-b)) bounds bond b. Current bond dimensions live on
+// the device (dims[0..n]); every kernel reads them there and the SVD kernel
+// writes the kept rank, so a whole Trotter step is enqueued without a single
+// host synchronisation.
+//
+// Two-site update (mps.cpp:41-100), three kernels:
+//   k_theta  theta[(a,r1),(r2,b)] = sum G[r1r2][s1s2] sum_c A(a,s1,c) C(c,s2,b),
+//            written column-major as M = theta (if 2Dr <= 2Dl) or theta^dagger.
+//   k_svd    one-sided Jacobi on the columns of a copy of M (FP64 angles,
+//            norms and convergence test), singular values = final column norms,
+//            sorted descending; keep = min(chi, k) minus trailing S < 1e-14
+//            (mps.cpp:80-82); discarded weight max-accumulated (mps.cpp:83-85).
+//   k_recon  A' = U[:, :keep], C' = U_keep^dagger theta  (= diag(S) Vh, mps.cpp:87-99).
+// The upstream computes its SVD from the Gram matrix (linalg.cpp:107-172); this
+// engine computes the same factorisation accurately, which is what the upstream
+// algorithm defines.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+
+#include "tebd.hpp"
+
+namespace tq {
+
+namespace {
+
+constexpr int kSvdThreads = 1024;
+constexpr int kMaxCols = 512;  // 2 * chi for chi <= 256
+constexpr double kTol = 1e-15;
+
+__device__ inline double2 ld_site(const double2* sites, std::size_t off, int stride, int a, int s, int b) {
+    return sites[off + ((std::size_t)a * 2 + s) * stride + b];
+}
+
+// ---- single-site half layer (mps.cpp:313-322) -----------------------------
+// exp(-i dt/2 g), g = [[-hz, -A], [-A, hz]], hz = B h_i (+ A bias_i), closed form.
+__global__ void __launch_bounds__(256) k_single_layer(double2* sites, const std::size_t* off, const int* cap,
+                                                      const int* dims, const double* h, const double* bias,
+                                                      double A, double B, double dt) {
+    const int i = blockIdx.x;
+    const int Dl = dims[i], Dr = dims[i + 1], stride = cap[i + 1];
+    const double hz = B * h[i] + (bias ? A * bias[i] : 0.0);
+    const double r = sqrt(A * A + hz * hz);
+    double2 u00 = make_double2(1, 0), u01 = make_double2(0, 0), u11 = make_double2(1, 0);
+    if (r > 0.0) {
+        double sn, cs;
+        sincos(0.5 * dt * r, &sn, &cs);
+        u00 = make_double2(cs, sn * hz / r);
+        u01 = make_double2(0.0, sn * A / r);
+        u11 = make_double2(cs, -sn * hz / r);
+    }
+    double2* t = sites + off[i];
+    for (int idx = threadIdx.x; idx < Dl * Dr; idx += blockDim.x) {
+        const int a = idx / Dr, b = idx % Dr;
+        double2* p0 = t + ((std::size_t)a * 2 + 0) * stride + b;
+        double2* p1 = t + ((std::size_t)a * 2 + 1) * stride + b;
+        const double2 v0 = *p0, v1 = *p1;
+        *p0 = cadd(cmul(u00, v0), cmul(u01, v1));
+        *p1 = cadd(cmul(u01, v0), cmul(u11, v1));
+    }
+}
+
+// ---- theta (mps.cpp:49-70) ------------------------------------------------
+__global__ void __launch_bounds__(256) k_theta(const double2* __restrict__ sites, const std::size_t* off,
+                                               const int* cap, const int* dims, int p, Gate4 g,
+                                               double2* __restrict__ M, double2* __restrict__ W, int ld) {
+    __shared__ double2 As[16][2][17];
+    __shared__ double2 Cs[16][2][17];
+    const int Dl = dims[p], mid = dims[p + 1], Dr = dims[p + 2];
+    const int sA = cap[p + 1], sC = cap[p + 2];
+    const std::size_t oA = off[p], oC = off[p + 1];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int a = blockIdx.y * 16 + ty, b = blockIdx.x * 16 + tx;
+    if (blockIdx.y * 16 >= Dl || blockIdx.x * 16 >= Dr) return;  // whole tile outside (uniform)
+    double2 th[2][2] = {{make_double2(0, 0), make_double2(0, 0)}, {make_double2(0, 0), make_double2(0, 0)}};
+    for (int c0 = 0; c0 < mid; c0 += 16) {
+        for (int s = 0; s < 2; ++s) {
+            const int aa = blockIdx.y * 16 + ty, cc = c0 + tx;
+            As[ty][s][tx] = (aa < Dl && cc < mid) ? ld_site(sites, oA, sA, aa, s, cc) : make_double2(0, 0);
+            const int cr = c0 + ty, bb = blockIdx.x * 16 + tx;
+            Cs[ty][s][tx] = (cr < mid && bb < Dr) ? ld_site(sites, oC, sC, cr, s, bb) : make_double2(0, 0);
+        }
+        __syncthreads();
+        for (int cc = 0; cc < 16; ++cc)
+            for (int s1 = 0; s1 < 2; ++s1) {
+                const double2 x = As[ty][s1][cc];
+                th[s1][0] = cadd(th[s1][0], cmul(x, Cs[cc][0][tx]));
+                th[s1][1] = cadd(th[s1][1], cmul(x, Cs[cc][1][tx]));
+            }
+        __syncthreads();
+    }
+    if (a >= Dl || b >= Dr) return;
+    const bool tall = Dr <= Dl;
+    for (int r1 = 0; r1 < 2; ++r1)
+        for (int r2 = 0; r2 < 2; ++r2) {
+            double2 acc = make_double2(0, 0);
+            for (int s1 = 0; s1 < 2; ++s1)
+                for (int s2 = 0; s2 < 2; ++s2) acc = cadd(acc, cmul(g.g[(r1 * 2 + r2) * 4 + s1 * 2 + s2], th[s1][s2]));
+            const int row = a * 2 + r1, col = r2 * Dr + b;
+            const std::size_t at = tall ? (std::size_t)col * ld + row : (std::size_t)row * ld + col;
+            const double2 v = tall ? acc : cconj(acc);
+            M[at] = v;
+            W[at] = v;
+        }
+}
+
+// ---- one-sided Jacobi SVD, one workgroup -----------------------------------
+__device__ inline void rr_pair(int r, int k, int cp, int& i, int& j) {
+    if (k == 0) {
+        i = r;
+        j = cp - 1;
+    } else {
+        i = (r + k) % (cp - 1);
+        j = (r - k + (cp - 1)) % (cp - 1);
+    }
+}
+
+__global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, int ld, int* dims, int p, int chi,
+                                                     double* __restrict__ sv_out, int* __restrict__ perm_out,
+                                                     double* trunc, int use_lds) {
+    extern __shared__ double2 lds[];
+    __shared__ double s_nrm[kMaxCols];
+    __shared__ int s_perm[kMaxCols];
+    __shared__ int s_flag;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int Dl = dims[p], Dr = dims[p + 2];
+    const int R = 2 * Dl, C = 2 * Dr;
+    const bool tall = C <= R;
+    const int m = tall ? R : C, c = tall ? C : R;
+    double2* W = Wg;
+    int lw = ld;
+    if (use_lds) {
+        W = lds;
+        lw = m;
+        for (int idx = tid; idx < m * c; idx += blockDim.x) {
+            const int col = idx / m, row = idx % m;
+            lds[idx] = Wg[(std::size_t)col * ld + row];
+        }
+        __syncthreads();
+    }
+    const int cp = c + (c & 1);
+    for (int sweep = 0; sweep < 60 && cp >= 2; ++sweep) {
+        if (tid == 0) s_flag = 0;
+        __syncthreads();
+        for (int r = 0; r < cp - 1; ++r) {
+            for (int k = wave; k < cp / 2; k += nwaves) {
+                int i, j;
+                rr_pair(r, k, cp, i, j);
+                if (i >= c || j >= c) continue;
+                double2* x = W + (std::size_t)i * lw;
+                double2* y = W + (std::size_t)j * lw;
+                double al = 0, be = 0, gr = 0, gi = 0;
+                for (int row = lane; row < m; row += 64) {
+                    const double2 u = x[row], v = y[row];
+                    al += u.x * u.x + u.y * u.y;
+                    be += v.x * v.x + v.y * v.y;
+                    gr += u.x * v.x + u.y * v.y;  // conj(u) v
+                    gi += u.x * v.y - u.y * v.x;
+                }
+                al = wave_sum(al);
+                be = wave_sum(be);
+                gr = wave_sum(gr);
+                gi = wave_sum(gi);
+                const double gabs = sqrt(gr * gr + gi * gi);
+                if (!(gabs > 0.0) || gabs <= kTol * sqrt(al * be)) continue;
+                if (lane == 0) s_flag = 1;
+                const double2 ph = make_double2(gr / gabs, -gi / gabs);  // conj(gamma)/|gamma|
+                const double zeta = (be - al) / (2.0 * gabs);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                for (int row = lane; row < m; row += 64) {
+                    const double2 u = x[row], v = cmul(y[row], ph);
+                    x[row] = make_double2(cs * u.x - sn * v.x, cs * u.y - sn * v.y);
+                    y[row] = make_double2(sn * u.x + cs * v.x, sn * u.y + cs * v.y);
+                }
+            }
+            __syncthreads();
+        }
+        const int f = s_flag;
+        __syncthreads();
+        if (!f) break;
+    }
+    // singular values = column norms (FP64)
+    for (int j = wave; j < c; j += nwaves) {
+        const double2* x = W + (std::size_t)j * lw;
+        double s = 0.0;
+        for (int row = lane; row < m; row += 64) s += x[row].x * x[row].x + x[row].y * x[row].y;
+        s = wave_sum(s);
+        if (lane == 0) s_nrm[j] = sqrt(s);
+    }
+    __syncthreads();
+    if (tid < c) {  // stable descending rank
+        const double v = s_nrm[tid];
+        int rank = 0;
+        for (int q = 0; q < c; ++q) {
+            const double u = s_nrm[q];
+            rank += (u > v) || (u == v && q < tid);
+        }
+        s_perm[rank] = tid;
+    }
+    __syncthreads();
+    if (tid == 0) {  // mps.cpp:77-85
+        double total = 0.0;
+        for (int q = 0; q < c; ++q) {
+            const double s = s_nrm[s_perm[q]];
+            total += s * s;
+        }
+        int keep = min(chi, c);
+        while (keep > 1 && s_nrm[s_perm[keep - 1]] < 1e-14) --keep;
+        double disc = 0.0;
+        for (int q = keep; q < c; ++q) {
+            const double s = s_nrm[s_perm[q]];
+            disc += s * s;
+        }
+        if (total > 0) {
+            const double tr = disc / total;
+            if (*trunc < tr) *trunc = tr;
+        }
+        dims[p + 1] = keep;
+    }
+    for (int q = tid; q < c; q += blockDim.x) {
+        sv_out[q] = s_nrm[s_perm[q]];
+        perm_out[q] = s_perm[q];
+    }
+    if (use_lds) {
+        for (int idx = tid; idx < m * c; idx += blockDim.x) {
+            const int col = idx / m, row = idx % m;
+            Wg[(std::size_t)col * ld + row] = lds[idx];
+        }
+    }
+}
+
+// ---- reconstruction (mps.cpp:87-99) ---------------------------------------
+// tall (M = theta):        A'[row,k] = W[:,perm k][row]/S_k;  C'[k,col] = sum_row conj(U[row,k]) M[row,col]
+// wide (M = theta^dagger): C'[k,col] = conj(W[col, perm k]);  A'[row,k] = sum_col conj(M[col,row]) W[col,perm k] / S_k^2
+__global__ void __launch_bounds__(256) k_recon(double2* sites, const std::size_t* off, const int* cap,
+                                               const int* dims, int p, const double2* __restrict__ M,
+                                               const double2* __restrict__ W, int ld,
+                                               const double* __restrict__ sv, const int* __restrict__ perm) {
+    const int Dl = dims[p], keep = dims[p + 1], Dr = dims[p + 2];
+    const int R = 2 * Dl, C = 2 * Dr;
+    const bool tall = C <= R;
+    const int sA = cap[p + 1], sC = cap[p + 2];
+    double2* A = sites + off[p];
+    double2* Cn = sites + off[p + 1];
+    const long long ndirect = tall ? (long long)R * keep : (long long)keep * C;
+    const long long ngemm = tall ? (long long)keep * C : (long long)R * keep;
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long idx = gid; idx < ndirect + ngemm; idx += stride) {
+        if (idx < ndirect) {
+            if (tall) {
+                const int row = (int)(idx / keep), k = (int)(idx % keep);
+                const double s = sv[k];
+                const double2 w = W[(std::size_t)perm[k] * ld + row];
+                A[(std::size_t)row * sA + k] = s > 0 ? cscale(w, 1.0 / s) : make_double2(0, 0);
+            } else {
+                const int k = (int)(idx / C), col = (int)(idx % C);
+                const int s2 = col / Dr, b = col % Dr;
+                Cn[((std::size_t)k * 2 + s2) * sC + b] = cconj(W[(std::size_t)perm[k] * ld + col]);
+            }
+        } else {
+            const long long g = idx - ndirect;
+            if (tall) {
+                const int k = (int)(g / C), col = (int)(g % C);
+                const double s = sv[k];
+                const double2* u = W + (std::size_t)perm[k] * ld;
+                const double2* mc = M + (std::size_t)col * ld;
+                double2 acc = make_double2(0, 0);
+                for (int row = 0; row < R; ++row) acc = cadd(acc, cmulc(u[row], mc[row]));
+                const int s2 = col / Dr, b = col % Dr;
+                Cn[((std::size_t)k * 2 + s2) * sC + b] = s > 0 ? cscale(acc, 1.0 / s) : make_double2(0, 0);
+            } else {
+                const int row = (int)(g / keep), k = (int)(g % keep);
+                const double s = sv[k];
+                const double2* w = W + (std::size_t)perm[k] * ld;
+                const double2* mr = M + (std::size_t)row * ld;
+                double2 acc = make_double2(0, 0);
+                for (int col = 0; col < C; ++col) acc = cadd(acc, cmulc(mr[col], w[col]));
+                A[(std::size_t)row * sA + k] = s > 0 ? cscale(acc, 1.0 / (s * s)) : make_double2(0, 0);
+            }
+        }
+    }
+}
+
+// ---- environments (mps.cpp:150-178, 218-262) ---------------------------------
+// Right: T[a',s,b] = sum_b' R[b,b'] conj(A(a',s,b'));  Rout[a,a'] = sum_{s,b} A(a,s,b) T[a',s,b]
+__global__ void __launch_bounds__(256) k_envr1(const double2* sites, const std::size_t* off, const int* cap,
+                                               const int* dims, int j, const double2* Rin, int ldr, double2* T) {
+    const int Dl = dims[j], Dr = dims[j + 1], st = cap[j + 1];
+    const std::size_t o = off[j];
+    const long long tot = (long long)Dl * 2 * Dr;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int ap = (int)(idx / (2 * Dr)), s = (int)((idx / Dr) % 2), b = (int)(idx % Dr);
+        double2 acc = make_double2(0, 0);
+        for (int bp = 0; bp < Dr; ++bp)
+            acc = cadd(acc, cmul(Rin[(std::size_t)b * ldr + bp], cconj(ld_site(sites, o, st, ap, s, bp))));
+        T[idx] = acc;
+    }
+}
+__global__ void __launch_bounds__(256) k_envr2(const double2* sites, const std::size_t* off, const int* cap,
+                                               const int* dims, int j, const double2* T, double2* Rout, int ldr) {
+    const int Dl = dims[j], Dr = dims[j + 1], st = cap[j + 1];
+    const std::size_t o = off[j];
+    const long long tot = (long long)Dl * Dl;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int a = (int)(idx / Dl), ap = (int)(idx % Dl);
+        double2 acc = make_double2(0, 0);
+        for (int s = 0; s < 2; ++s)
+            for (int b = 0; b < Dr; ++b)
+                acc = cadd(acc, cmul(ld_site(sites, o, st, a, s, b), T[((std::size_t)ap * 2 + s) * Dr + b]));
+        Rout[(std::size_t)a * ldr + ap] = acc;
+    }
+}
+// Left: T[a,s,b'] = sum_a' L[a,a'] conj(A(a',s,b'));  Lout[b,b'] = sum_{s,a} zf(s) A(a,s,b) T[a,s,b']
+__global__ void __launch_bounds__(256) k_envl1(const double2* sites, const std::size_t* off, const int* cap,
+                                               const int* dims, int j, const double2* Lin, int ldl, double2* T) {
+    const int Dl = dims[j], Dr = dims[j + 1], st = cap[j + 1];
+    const std::size_t o = off[j];
+    const long long tot = (long long)Dl * 2 * Dr;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int a = (int)(idx / (2 * Dr)), s = (int)((idx / Dr) % 2), bp = (int)(idx % Dr);
+        double2 acc = make_double2(0, 0);
+        for (int ap = 0; ap < Dl; ++ap)
+            acc = cadd(acc, cmul(Lin[(std::size_t)a * ldl + ap], cconj(ld_site(sites, o, st, ap, s, bp))));
+        T[idx] = acc;
+    }
+}
+__global__ void __launch_bounds__(256) k_envl2(const double2* sites, const std::size_t* off, const int* cap,
+                                               const int* dims, int j, int withz, const double2* T, double2* Lout,
+                                               int ldl) {
+    const int Dl = dims[j], Dr = dims[j + 1], st = cap[j + 1];
+    const std::size_t o = off[j];
+    const long long tot = (long long)Dr * Dr;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(idx / Dr), bp = (int)(idx % Dr);
+        double2 acc = make_double2(0, 0);
+        for (int s = 0; s < 2; ++s) {
+            double2 part = make_double2(0, 0);
+            for (int a = 0; a < Dl; ++a)
+                part = cadd(part, cmul(ld_site(sites, o, st, a, s, b), T[((std::size_t)a * 2 + s) * Dr + bp]));
+            acc = (withz && s == 1) ? csub(acc, part) : cadd(acc, part);
+        }
+        Lout[(std::size_t)b * ldl + bp] = acc;
+    }
+}
+
+__global__ void k_set_unit(double2* e) { e[0] = make_double2(1.0, 0.0); }
+
+// Qubit-0 reduced density matrix from the right environment of site 1 and the
+// closed-form 2x2 entropy (mps.cpp:263-290).
+__global__ void k_rho0(const double2* sites, const std::size_t* off, const int* cap, const int* dims,
+                       const double2* R1, int ldr, double* out, int slot) {
+    if (threadIdx.x != 0) return;
+    const int Dr = dims[1], st = cap[1];
+    double2 rho[2][2];
+    for (int s = 0; s < 2; ++s)
+        for (int sp = 0; sp < 2; ++sp) {
+            double2 acc = make_double2(0, 0);
+            for (int b = 0; b < Dr; ++b)
+                for (int bp = 0; bp < Dr; ++bp)
+                    acc = cadd(acc, cmul(cmul(ld_site(sites, off[0], st, 0, s, b),
+                                              cconj(ld_site(sites, off[0], st, 0, sp, bp))),
+                                         R1[(std::size_t)b * ldr + bp]));
+            rho[s][sp] = acc;
+        }
+    const double tr = rho[0][0].x + rho[1][1].x;
+    double ent = 0.0;
+    if (tr > 0) {
+        const double a = rho[0][0].x / tr, d = rho[1][1].x / tr;
+        const double2 o = make_double2(rho[0][1].x / tr, rho[0][1].y / tr);
+        const double off2 = o.x * o.x + o.y * o.y;
+        const double mean = (a + d) / 2.0;
+        const double rad = sqrt(fmax(0.0, ((a - d) / 2) * ((a - d) / 2) + off2));
+        const double l1 = fmin(1.0, fmax(0.0, mean + rad)), l2 = fmin(1.0, fmax(0.0, mean - rad));
+        if (l1 > 1e-12) ent -= l1 * log2(l1);
+        if (l2 > 1e-12) ent -= l2 * log2(l2);
+    }
+    out[slot] = ent;
+}
+
+// sum_{b,b'} X[b,b'] R[b,b'] over a d x d block -> out[slot]
+__global__ void __launch_bounds__(256) k_close(const double2* X, const double2* Rm, int ld, const int* dims,
+                                               int bond, double2* out, int slot) {
+    __shared__ double red[4][2];
+    const int d = dims[bond];
+    double re = 0, im = 0;
+    for (int idx = threadIdx.x; idx < d * d; idx += blockDim.x) {
+        const int b = idx / d, bp = idx % d;
+        const double2 v = cmul(X[(std::size_t)b * ld + bp], Rm[(std::size_t)b * ld + bp]);
+        re += v.x;
+        im += v.y;
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6][0] = re;
+        red[threadIdx.x >> 6][1] = im;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            a += red[w][0];
+            b += red[w][1];
+        }
+        out[slot] = make_double2(a, b);
+    }
+}
+
+// <x|psi> for a fixed configuration (mps.cpp:197-213), one workgroup.
+__global__ void __launch_bounds__(256) k_amplitude(const double2* sites, const std::size_t* off, const int* cap,
+                                                   const int* dims, const int8_t* spins, int n, double2* out) {
+    __shared__ double2 v[2][256];
+    if (threadIdx.x == 0) v[0][0] = make_double2(1, 0);
+    __syncthreads();
+    int cur = 0;
+    for (int i = 0; i < n; ++i) {
+        const int Dl = dims[i], Dr = dims[i + 1], st = cap[i + 1];
+        const int s = spins[i] > 0 ? 0 : 1;
+        for (int b = threadIdx.x; b < Dr; b += blockDim.x) {
+            double2 acc = make_double2(0, 0);
+            for (int a = 0; a < Dl; ++a) acc = cadd(acc, cmul(v[cur][a], ld_site(sites, off[i], st, a, s, b)));
+            v[cur ^ 1][b] = acc;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    if (threadIdx.x == 0) *out = v[cur][0];
+}
+
+// Random saturated state: Gaussian entries from a counter hash, scaled by
+// exp(-b/decay) on the right bond index and 1/sqrt(2 Dr) per site.
+__device__ inline double u01(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return ((x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+__global__ void k_randomize(double2* sites, const std::size_t* off, const int* cap, int n, unsigned long long seed,
+                            double decay) {
+    const int i = blockIdx.y;
+    const int Dl = cap[i], Dr = cap[i + 1];
+    const double norm = 1.0 / sqrt(2.0 * Dr);
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < Dl * 2 * Dr; idx += gridDim.x * blockDim.x) {
+        const int b = idx % Dr;
+        const unsigned long long key = seed * 0x100000001B3ull + (unsigned long long)off[i] + idx;
+        const double u1 = u01(2 * key), u2 = u01(2 * key + 1);
+        const double rad = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincos(6.283185307179586 * u2, &sn, &cs);
+        const double w = norm * exp(-b / decay);
+        sites[off[i] + idx] = make_double2(rad * cs * w, rad * sn * w);
+    }
+}
+
+int64_t grid_for(long long work, int block) {
+    long long g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 65535) g = 65535;
+    return g;
+}
+
+}  // namespace
+
+Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
+    : n_(model.n), chi_(chi), ns_(zz_substeps < 1 ? 1 : zz_substeps), dt_(dt), model_(model) {
+    if (n_ < 1) throw Error(-1, "TEBD needs n >= 1");
+    if (chi_ < 1) throw Error(-1, "TEBD needs chi >= 1");
+    if (chi_ > 256) throw Error(-2, "this build supports chi <= 256");
+    for (std::size_t e = 0; e < model_.ew.size(); ++e)
+        if (model_.ei[e] < 0 || model_.ej[e] < 0 || model_.ei[e] >= n_ || model_.ej[e] >= n_)
+            throw Error(-1, "edge index out of range");
+    cap_.resize(n_ + 1);
+    for (int b = 0; b <= n_; ++b) {
+        const int e = std::min(b, n_ - b);
+        cap_[b] = e >= 30 ? chi_ : std::min<long long>(chi_, 1ll << e);
+    }
+    off_.resize(n_ + 1);
+    std::size_t tot = 0;
+    for (int i = 0; i < n_; ++i) {
+        off_[i] = tot;
+        tot += (std::size_t)cap_[i] * 2 * cap_[i + 1];
+    }
+    off_[n_] = tot;
+    TQ_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    sites_.alloc(tot);
+    dims_.alloc(n_ + 1);
+    int maxcap = 1;
+    for (int c : cap_) maxcap = std::max(maxcap, c);
+    maxcap_ = maxcap;
+    const int ld = 2 * maxcap;
+    mat_.alloc((std::size_t)ld * ld);
+    work_.alloc((std::size_t)ld * ld);
+    sv_.alloc(ld);
+    perm_.alloc(ld);
+    trunc_.alloc(1);
+    h_.alloc(n_);
+    TQ_HIP(hipMemcpy(h_.p, model_.h.data(), n_ * sizeof(double), hipMemcpyHostToDevice));
+    if (!model_.bias.empty()) {
+        bias_.alloc(n_);
+        TQ_HIP(hipMemcpy(bias_.p, model_.bias.data(), n_ * sizeof(double), hipMemcpyHostToDevice));
+    }
+    envA_.alloc((std::size_t)maxcap * maxcap);
+    envB_.alloc((std::size_t)maxcap * maxcap);
+    envT_.alloc((std::size_t)maxcap * 2 * maxcap);
+    // device copies of the layout tables live right after the dims array
+    caps_dev_.alloc(n_ + 1);
+    off_dev_.alloc(n_ + 1);
+    TQ_HIP(hipMemcpy(caps_dev_.p, cap_.data(), (n_ + 1) * sizeof(int), hipMemcpyHostToDevice));
+    TQ_HIP(hipMemcpy(off_dev_.p, off_.data(), (n_ + 1) * sizeof(std::size_t), hipMemcpyHostToDevice));
+    TQ_HIP(hipFuncSetAttribute((const void*)k_svd, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
+    build_plan();
+    reset_product();
+}
+
+Tebd::~Tebd() {
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// mps.cpp:126-145 routing of one ZZ slice in edge-list order.
+void Tebd::build_plan() {
+    plan_.clear();
+    spans_.clear();
+    for (std::size_t e = 0; e < model_.ew.size(); ++e) {
+        int i = model_.ei[e], j = model_.ej[e];
+        if (i > j) std::swap(i, j);
+        if (i == j) continue;
+        for (int k = j; k > i + 1; --k) plan_.push_back({k - 1, -1});
+        plan_.push_back({i, (int)e});
+        for (int k = i + 1; k < j; ++k) plan_.push_back({k, -1});
+        spans_.push_back({plan_.size() - (std::size_t)(2 * (j - i) - 1), plan_.size()});
+    }
+}
+
+void Tebd::ensure_trace(int count) {
+    if (count > trace_cap_) {
+        DevBuf<double> nb(count);
+        if (trace_cap_) TQ_HIP(hipMemcpyAsync(nb.p, trace_.p, trace_cap_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        TQ_HIP(hipStreamSynchronize(stream_));
+        trace_ = std::move(nb);
+        trace_cap_ = count;
+    }
+}
+
+void Tebd::reset_product() {
+    std::vector<double2> host(off_[n_], make_double2(0, 0));
+    const double amp = 1.0 / std::sqrt(2.0);
+    for (int i = 0; i < n_; ++i) {
+        host[off_[i] + 0 * cap_[i + 1]] = make_double2(amp, 0);  // (0,0,0)
+        host[off_[i] + 1 * cap_[i + 1]] = make_double2(amp, 0);  // (0,1,0)
+    }
+    TQ_HIP(hipMemcpyAsync(sites_.p, host.data(), host.size() * sizeof(double2), hipMemcpyHostToDevice, stream_));
+    std::vector<int> ones(n_ + 1, 1);
+    TQ_HIP(hipMemcpyAsync(dims_.p, ones.data(), ones.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+    const double zero = 0.0;
+    TQ_HIP(hipMemcpyAsync(trunc_.p, &zero, sizeof(double), hipMemcpyHostToDevice, stream_));
+    TQ_HIP(hipStreamSynchronize(stream_));
+}
+
+void Tebd::randomize(uint64_t seed, double decay) {
+    int maxsite = 1;
+    for (int i = 0; i < n_; ++i) maxsite = std::max(maxsite, cap_[i] * 2 * cap_[i + 1]);
+    dim3 grid((unsigned)std::min<long long>((maxsite + 255) / 256, 1024), n_);
+    hipLaunchKernelGGL(k_randomize, grid, dim3(256), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, n_,
+                       (unsigned long long)seed, decay);
+    TQ_LAUNCH_CHECK();
+    TQ_HIP(hipMemcpyAsync(dims_.p, cap_.data(), (n_ + 1) * sizeof(int), hipMemcpyHostToDevice, stream_));
+    const double zero = 0.0;
+    TQ_HIP(hipMemcpyAsync(trunc_.p, &zero, sizeof(double), hipMemcpyHostToDevice, stream_));
+    TQ_HIP(hipStreamSynchronize(stream_));
+}
+
+void Tebd::two_site(int p, const Gate4& g) {
+    const int ld = (int)std::lround(std::sqrt((double)mat_.n));
+    const int Dlc = cap_[p], Drc = cap_[p + 2];
+    dim3 tgrid((Drc + 15) / 16, (Dlc + 15) / 16);
+    hipLaunchKernelGGL(k_theta, tgrid, dim3(16, 16), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p, p, g,
+                       mat_.p, work_.p, ld);
+    TQ_LAUNCH_CHECK();
+    // worst-case matrix at this bond decides LDS residency (both dims <= 2*cap)
+    const std::size_t bytes = (std::size_t)(2 * Dlc) * (2 * Drc) * sizeof(double2);
+    const int use_lds = bytes <= 144 * 1024 ? 1 : 0;
+    hipLaunchKernelGGL(k_svd, dim3(1), dim3(kSvdThreads), use_lds ? bytes : 0, stream_, work_.p, ld, dims_.p, p,
+                       chi_, sv_.p, perm_.p, trunc_.p, use_lds);
+    TQ_LAUNCH_CHECK();
+    const long long work = (long long)(2 * Dlc) * std::min(chi_, 2 * Drc) + (long long)std::min(chi_, 2 * Dlc) * (2 * Drc);
+    hipLaunchKernelGGL(k_recon, dim3((unsigned)grid_for(work, 256)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
+                       caps_dev_.p, dims_.p, p, mat_.p, work_.p, ld, sv_.p, perm_.p);
+    TQ_LAUNCH_CHECK();
+}
+
+static Gate4 make_swap_gate() {
+    Gate4 g;
+    for (auto& x : g.g) x = make_double2(0, 0);
+    g.g[0 * 4 + 0] = make_double2(1, 0);
+    g.g[1 * 4 + 2] = make_double2(1, 0);
+    g.g[2 * 4 + 1] = make_double2(1, 0);
+    g.g[3 * 4 + 3] = make_double2(1, 0);
+    return g;
+}
+
+// exp(-i theta Z(x)Z) = diag(e^{-i th}, e^{i th}, e^{i th}, e^{-i th})  (mps.cpp:116-124)
+static Gate4 make_zz_gate(double theta) {
+    Gate4 g;
+    for (auto& x : g.g) x = make_double2(0, 0);
+    const double c = std::cos(theta), s = std::sin(theta);
+    g.g[0] = make_double2(c, -s);
+    g.g[5] = make_double2(c, s);
+    g.g[10] = make_double2(c, s);
+    g.g[15] = make_double2(c, -s);
+    return g;
+}
+
+void Tebd::step(double A, double B, int rec) {
+    const double* bias = bias_.p;
+    hipLaunchKernelGGL(k_single_layer, dim3(n_), dim3(256), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p,
+                       h_.p, bias, A, B, dt_);
+    TQ_LAUNCH_CHECK();
+    static const Gate4 sw = make_swap_gate();
+    const double sub = dt_ / ns_;
+    auto run_edge = [&](const std::pair<std::size_t, std::size_t>& span) {
+        for (std::size_t q = span.first; q < span.second; ++q) {
+            const Update& u = plan_[q];
+            two_site(u.site, u.edge < 0 ? sw : make_zz_gate(-B * model_.ew[u.edge] * sub));
+        }
+    };
+    for (int s = 0; s < ns_; ++s) {
+        // odd slices visit the edge list backwards, each edge's own swap-in /
+        // gate / swap-out sequence unchanged (mps.cpp:333-340)
+        if (s % 2 == 0)
+            for (const auto& span : spans_) run_edge(span);
+        else
+            for (std::size_t k = spans_.size(); k-- > 0;) run_edge(spans_[k]);
+    }
+    hipLaunchKernelGGL(k_single_layer, dim3(n_), dim3(256), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p,
+                       h_.p, bias, A, B, dt_);
+    TQ_LAUNCH_CHECK();
+    if (rec >= 0) entropy_sweep(rec);
+}
+
+void Tebd::env_right(const double2* rin, int j, double2* rout) {
+    const int ldr = chi_max();
+    const long long w1 = (long long)cap_[j] * 2 * cap_[j + 1];
+    hipLaunchKernelGGL(k_envr1, dim3((unsigned)grid_for(w1, 256)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
+                       caps_dev_.p, dims_.p, j, rin, ldr, envT_.p);
+    TQ_LAUNCH_CHECK();
+    const long long w2 = (long long)cap_[j] * cap_[j];
+    hipLaunchKernelGGL(k_envr2, dim3((unsigned)grid_for(w2, 256)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
+                       caps_dev_.p, dims_.p, j, envT_.p, rout, ldr);
+    TQ_LAUNCH_CHECK();
+}
+
+void Tebd::env_left(const double2* lin, int j, bool z, double2* lout) {
+    const int ldl = chi_max();
+    const long long w1 = (long long)cap_[j] * 2 * cap_[j + 1];
+    hipLaunchKernelGGL(k_envl1, dim3((unsigned)grid_for(w1, 256)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
+                       caps_dev_.p, dims_.p, j, lin, ldl, envT_.p);
+    TQ_LAUNCH_CHECK();
+    const long long w2 = (long long)cap_[j + 1] * cap_[j + 1];
+    hipLaunchKernelGGL(k_envl2, dim3((unsigned)grid_for(w2, 256)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
+                       caps_dev_.p, dims_.p, j, z ? 1 : 0, envT_.p, lout, ldl);
+    TQ_LAUNCH_CHECK();
+}
+
+// mps.cpp:355 -> single_site_entropy(0): right environment over sites n-1..1.
+void Tebd::entropy_sweep(int rec) {
+    ensure_trace(rec + 1);
+    if (n_ < 2) {
+        const double zero = 0.0;
+        TQ_HIP(hipMemcpyAsync(trace_.p + rec, &zero, sizeof(double), hipMemcpyHostToDevice, stream_));
+        TQ_HIP(hipStreamSynchronize(stream_));
+        return;
+    }
+    double2* cur = envA_.p;
+    double2* nxt = envB_.p;
+    hipLaunchKernelGGL(k_set_unit, dim3(1), dim3(1), 0, stream_, cur);
+    TQ_LAUNCH_CHECK();
+    for (int j = n_ - 1; j >= 1; --j) {
+        env_right(cur, j, nxt);
+        std::swap(cur, nxt);
+    }
+    hipLaunchKernelGGL(k_rho0, dim3(1), dim3(64), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p, cur,
+                       chi_max(), trace_.p, rec);
+    TQ_LAUNCH_CHECK();
+}
+
+void Tebd::sync() { TQ_HIP(hipStreamSynchronize(stream_)); }
+
+std::vector<double> Tebd::entropy(int count) {
+    std::vector<double> out(count, 0.0);
+    if (count > 0 && trace_cap_ >= count)
+        TQ_HIP(hipMemcpyAsync(out.data(), trace_.p, count * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    return out;
+}
+
+double Tebd::max_trunc() {
+    double t = 0.0;
+    TQ_HIP(hipMemcpyAsync(&t, trunc_.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    return t;
+}
+
+std::vector<int> Tebd::bonds() {
+    std::vector<int> d(n_ + 1);
+    TQ_HIP(hipMemcpyAsync(d.data(), dims_.p, (n_ + 1) * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    sync();
+    return d;
+}
+
+// mps.cpp:358-380: <H> from normalised <Z_i Z_j> and <Z_i>, spins = sign<Z_i>
+// (0 -> +1), readout energy in edge-then-field order, prob = |<x|psi>|^2 / norm.
+Tebd::Readout Tebd::readout() {
+    const int ld = chi_max();
+    const std::size_t blk = (std::size_t)ld * ld;
+    DevBuf<double2> L((std::size_t)(n_ + 1) * blk), Rr((std::size_t)(n_ + 1) * blk), X(blk), Y(blk);
+    hipLaunchKernelGGL(k_set_unit, dim3(1), dim3(1), 0, stream_, L.p);
+    hipLaunchKernelGGL(k_set_unit, dim3(1), dim3(1), 0, stream_, Rr.p + (std::size_t)n_ * blk);
+    TQ_LAUNCH_CHECK();
+    for (int i = 0; i < n_; ++i) env_left(L.p + (std::size_t)i * blk, i, false, L.p + (std::size_t)(i + 1) * blk);
+    for (int j = n_ - 1; j >= 0; --j) env_right(Rr.p + (std::size_t)(j + 1) * blk, j, Rr.p + (std::size_t)j * blk);
+    const int m = (int)model_.ew.size();
+    // slots: 0 norm, 1..n <Z_i>, n+1..n+m <ZZ_e>, n+m+1 amplitude
+    DevBuf<double2> res(n_ + m + 2);
+    hipLaunchKernelGGL(k_close, dim3(1), dim3(256), 0, stream_, L.p + (std::size_t)n_ * blk, Rr.p + (std::size_t)n_ * blk,
+                       ld, dims_.p, n_, res.p, 0);
+    for (int i = 0; i < n_; ++i) {
+        env_left(L.p + (std::size_t)i * blk, i, true, X.p);
+        hipLaunchKernelGGL(k_close, dim3(1), dim3(256), 0, stream_, X.p, Rr.p + (std::size_t)(i + 1) * blk, ld,
+                           dims_.p, i + 1, res.p, 1 + i);
+    }
+    for (int e = 0; e < m; ++e) {
+        int i = model_.ei[e], j = model_.ej[e];
+        if (i > j) std::swap(i, j);
+        double2* a = X.p;
+        double2* b = Y.p;
+        env_left(L.p + (std::size_t)i * blk, i, true, a);
+        for (int k = i + 1; k <= j; ++k) {
+            env_left(a, k, k == j, b);
+            std::swap(a, b);
+        }
+        // i == j: a single Z at i, as the upstream z-string membership test does
+        hipLaunchKernelGGL(k_close, dim3(1), dim3(256), 0, stream_, a, Rr.p + (std::size_t)(j + 1) * blk, ld, dims_.p,
+                           j + 1, res.p, 1 + n_ + e);
+    }
+    TQ_LAUNCH_CHECK();
+    std::vector<double2> hres(n_ + m + 2);
+    TQ_HIP(hipMemcpyAsync(hres.data(), res.p, (n_ + m + 1) * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    sync();
+    const double nrm = hres[0].x;
+    Readout out;
+    std::vector<double> z(n_), zz(m);
+    for (int i = 0; i < n_; ++i) z[i] = nrm <= 0 ? 0.0 : hres[1 + i].x / nrm;
+    for (int e = 0; e < m; ++e) zz[e] = nrm <= 0 ? 0.0 : hres[1 + n_ + e].x / nrm;
+    double eexp = 0.0;
+    for (int e = 0; e < m; ++e) eexp -= model_.ew[e] * zz[e];
+    for (int i = 0; i < n_; ++i) eexp -= model_.h[i] * z[i];
+    out.expected_energy = eexp;
+    out.spins.resize(n_);
+    for (int i = 0; i < n_; ++i) out.spins[i] = z[i] >= 0.0 ? 1 : -1;
+    double en = 0.0;
+    for (int e = 0; e < m; ++e) en -= model_.ew[e] * out.spins[model_.ei[e]] * out.spins[model_.ej[e]];
+    for (int i = 0; i < n_; ++i) en -= model_.h[i] * out.spins[i];
+    out.energy = en;
+    DevBuf<int8_t> dsp(n_);
+    TQ_HIP(hipMemcpyAsync(dsp.p, out.spins.data(), n_, hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(k_amplitude, dim3(1), dim3(256), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p, dsp.p,
+                       n_, res.p + n_ + m + 1);
+    TQ_LAUNCH_CHECK();
+    double2 amp;
+    TQ_HIP(hipMemcpyAsync(&amp, res.p + n_ + m + 1, sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    sync();
+    out.prob = nrm > 0 ? (amp.x * amp.x + amp.y * amp.y) / nrm : 0.0;
+    return out;
+}
+
+std::vector<double> Tebd::dense() {
+    if (n_ > 24) throw Error(-1, "dense state only for n <= 24");
+    std::vector<double2> host(off_[n_]);
+    TQ_HIP(hipMemcpyAsync(host.data(), sites_.p, host.size() * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    std::vector<int> d = bonds();
+    std::vector<std::complex<double>> cur(1, {1.0, 0.0});
+    std::size_t rows = 1;
+    int bond = 1;
+    for (int i = 0; i < n_; ++i) {
+        const int Dr = d[i + 1], st = cap_[i + 1];
+        std::vector<std::complex<double>> nxt(rows * 2 * Dr);
+        for (std::size_t x = 0; x < rows; ++x)
+            for (int s = 0; s < 2; ++s)
+                for (int b = 0; b < Dr; ++b) {
+                    std::complex<double> acc(0, 0);
+                    for (int a = 0; a < bond; ++a) {
+                        const double2 t = host[off_[i] + ((std::size_t)a * 2 + s) * st + b];
+                        acc += cur[x * bond + a] * std::complex<double>(t.x, t.y);
+                    }
+                    nxt[(x + ((std::size_t)s << i)) * Dr + b] = acc;
+                }
+        cur.swap(nxt);
+        rows *= 2;
+        bond = Dr;
+    }
+    std::vector<double> out(2 * cur.size());
+    for (std::size_t x = 0; x < cur.size(); ++x) {
+        out[2 * x] = cur[x].real();
+        out[2 * x + 1] = cur[x].imag();
+    }
+    return out;
+}
+
+}  // namespace tq

@@ -1,0 +1,89 @@
+// MPS-TEBD engine on gfx950 (upstream cuda/src/mps.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+
+namespace tq {
+
+struct TebdModel {
+    int n = 0;
+    std::vector<double> h;
+    std::vector<int> ei, ej;  // as given; routing normalises i < j (mps.cpp:127)
+    std::vector<double> ew;
+    std::vector<double> bias;  // empty or n entries
+};
+
+// One two-site update of the routed ZZ block.
+struct Update {
+    int site;   // acts on (site, site+1)
+    int edge;   // -1 for SWAP, else index of the ZZ edge
+};
+
+class Tebd {
+public:
+    Tebd(const TebdModel& model, int chi, double dt, int zz_substeps);
+    ~Tebd();
+    Tebd(const Tebd&) = delete;
+    Tebd& operator=(const Tebd&) = delete;
+
+    void reset_product();                       // |+>^n
+    void randomize(uint64_t seed, double decay);  // saturated random state
+    // Enqueue one Trotter step (mps.cpp:309-356); entropy recorded at slot `rec` if >= 0.
+    void step(double A, double B, int rec);
+    void sync();
+    std::vector<double> entropy(int count);      // recorded slots [0, count)
+    double max_trunc();
+    std::vector<int> bonds();                    // n+1 bond dims
+    struct Readout {
+        std::vector<int8_t> spins;
+        double energy = 0, expected_energy = 0, prob = 0;
+    };
+    Readout readout();                           // mps.cpp:358-380
+    std::vector<double> dense();                 // n <= 24, interleaved re/im
+    std::size_t updates_per_step() const { return plan_.size() * (std::size_t)ns_; }
+    int n() const { return n_; }
+    hipStream_t stream() const { return stream_; }
+    // Capacity of bond b (0..n): min(chi, 2^b, 2^(n-b)).
+    int cap(int b) const { return cap_[b]; }
+    void ensure_trace(int count);
+
+private:
+    void build_plan();
+    void two_site(int p, const Gate4& g);
+    void entropy_sweep(int rec);
+    void env_left(const double2* lin, int site, bool z, double2* lout);
+    void env_right(const double2* rin, int site, double2* rout);
+    double2* site_ptr(int i) const { return sites_.p + off_[i]; }
+    int chi_max() const { return maxcap_; }
+
+    int n_, chi_, ns_;
+    double dt_;
+    TebdModel model_;
+    std::vector<int> cap_;
+    std::vector<std::size_t> off_;
+    std::vector<Update> plan_;  // one ZZ slice in forward order
+    std::vector<std::pair<std::size_t, std::size_t>> spans_;  // plan_ range of each edge
+    int maxcap_ = 1;
+    hipStream_t stream_ = nullptr;
+
+    DevBuf<double2> sites_;
+    DevBuf<int> dims_;
+    DevBuf<int> caps_dev_;
+    DevBuf<std::size_t> off_dev_;
+    DevBuf<double2> mat_, work_;   // theta (kept) and the Jacobi working copy
+    DevBuf<double> sv_;            // sorted singular values
+    DevBuf<int> perm_;             // sorted position -> column
+    DevBuf<double> trunc_;         // running max discarded weight
+    DevBuf<double> h_, bias_;
+    DevBuf<double2> envA_, envB_, envT_;
+    DevBuf<double> trace_;
+    int trace_cap_ = 0;
+    std::size_t svd_lds_ = 0;
+};
+
+}  // namespace tq

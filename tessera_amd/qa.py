@@ -1,0 +1,285 @@
+"""Host-side mirror of TESSERA's annealing interface over libtessera_qa.so.
+
+Names and semantics follow the upstream callers of the hot path:
+  quantum_anneal(model, sched, chi)      rust/src/ffi.rs:47-85  (QaError, QaResult)
+  anneal_statevector(model, sched, dt)   rust/src/quantum.rs:63-112 (QaOutcome)
+  solve_qa(h, ei, ej, ew, steps, dt)     rust/src/capi.rs:107-143
+plus the extended MPS entry point (mps_anneal's full AnnealResult,
+cuda/include/mps.hpp:37-44,76-82) and the native TEBD session.
+
+The library is the product path: nothing here falls back to a CPU
+implementation. If libtessera_qa.so is missing or no GPU is present the calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import math
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from .instances import Ising, Schedule
+
+_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libtessera_qa.so"
+_lib = None
+
+_f64p = ctypes.POINTER(ctypes.c_double)
+_usz = ctypes.c_size_t
+
+
+class TesseraError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"tessera_qa status {status}: {msg}")
+        self.status = status
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    """Load libtessera_qa.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise RuntimeError(f"{_LIB_PATH} is missing: build it with `python -m tessera_amd.build`")
+        L = ctypes.CDLL(str(_LIB_PATH))
+        vp = ctypes.c_void_p
+        L.tessera_qa_anneal.restype = ctypes.c_double
+        L.tessera_qa_anneal.argtypes = [vp, _usz, vp, vp, vp, _usz, vp, vp, _usz, _usz, vp, vp]
+        L.tessera_qa_anneal_ex.restype = ctypes.c_int
+        L.tessera_qa_anneal_ex.argtypes = [vp, _usz, vp, vp, vp, _usz, vp, vp, _usz, ctypes.c_double, _usz,
+                                           ctypes.c_int, vp, vp, vp, vp, vp]
+        L.tessera_sv_anneal.restype = ctypes.c_int
+        L.tessera_sv_anneal.argtypes = [vp, _usz, vp, vp, vp, _usz, vp, vp, _usz, ctypes.c_double, vp, vp, vp, vp,
+                                        vp, vp]
+        L.tessera_solve_qa.restype = ctypes.c_double
+        L.tessera_solve_qa.argtypes = [vp, _usz, vp, vp, vp, _usz, _usz, ctypes.c_double, vp, vp, vp, vp]
+        L.tessera_tebd_create.restype = ctypes.c_int
+        L.tessera_tebd_create.argtypes = [vp, _usz, vp, vp, vp, _usz, _usz, ctypes.c_double, ctypes.c_int,
+                                          ctypes.POINTER(vp)]
+        L.tessera_tebd_reset.restype = ctypes.c_int
+        L.tessera_tebd_reset.argtypes = [vp]
+        L.tessera_tebd_randomize.restype = ctypes.c_int
+        L.tessera_tebd_randomize.argtypes = [vp, ctypes.c_uint64, ctypes.c_double]
+        L.tessera_tebd_run.restype = ctypes.c_int
+        L.tessera_tebd_run.argtypes = [vp, vp, vp, _usz, ctypes.c_int]
+        L.tessera_tebd_sync.restype = ctypes.c_int
+        L.tessera_tebd_sync.argtypes = [vp, vp, _usz, vp, vp]
+        L.tessera_tebd_readout.restype = ctypes.c_int
+        L.tessera_tebd_readout.argtypes = [vp, vp, vp]
+        L.tessera_tebd_dense.restype = ctypes.c_int
+        L.tessera_tebd_dense.argtypes = [vp, vp]
+        L.tessera_tebd_updates_per_step.restype = _usz
+        L.tessera_tebd_updates_per_step.argtypes = [vp]
+        L.tessera_tebd_destroy.restype = None
+        L.tessera_tebd_destroy.argtypes = [vp]
+        L.tessera_last_error.restype = ctypes.c_char_p
+        L.tessera_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(status: int):
+    if status != 0:
+        raise TesseraError(status, lib().tessera_last_error().decode())
+
+
+def _edges(model: Ising):
+    ei = np.ascontiguousarray(model.ei, dtype=np.uintp)
+    ej = np.ascontiguousarray(model.ej, dtype=np.uintp)
+    ew = np.ascontiguousarray(model.ew, dtype=np.float64)
+    return ei, ej, ew
+
+
+# ---------------------------------------------------------------- ffi.rs mirror
+class QaError(enum.Enum):
+    """rust/src/ffi.rs:12-17. EngineNotLinked has no analogue here: a missing
+    library raises instead."""
+
+    EngineNotLinked = "EngineNotLinked"
+    BadSchedule = "BadSchedule"
+
+
+class QaFailure(Exception):
+    def __init__(self, kind: QaError):
+        super().__init__(kind.value)
+        self.kind = kind
+
+
+@dataclass
+class QaResult:
+    """rust/src/ffi.rs:39-43"""
+
+    spins: np.ndarray
+    energy: float
+    entropy: np.ndarray
+
+
+def quantum_anneal(model: Ising, sched: Schedule, chi: int) -> QaResult:
+    """rust/src/ffi.rs:47-85: MPS-TEBD through tessera_qa_anneal (dt = 0.1)."""
+    steps = sched.steps
+    if steps < 2:
+        raise QaFailure(QaError.BadSchedule)
+    ei, ej, ew = _edges(model)
+    spins = np.zeros(model.n, np.int8)
+    entropy = np.zeros(steps, np.float64)
+    a = np.ascontiguousarray(sched.a, np.float64)
+    b = np.ascontiguousarray(sched.b, np.float64)
+    energy = lib().tessera_qa_anneal(_p(model.h), model.n, _p(ei), _p(ej), _p(ew), model.m, _p(a), _p(b), steps,
+                                     chi, _p(spins), _p(entropy))
+    if math.isnan(energy):
+        raise TesseraError(-3, lib().tessera_last_error().decode())
+    return QaResult(spins=spins, energy=energy, entropy=entropy)
+
+
+@dataclass
+class AnnealResult:
+    """cuda/include/mps.hpp:37-44 plus the per-step largest bond."""
+
+    spins: np.ndarray
+    energy: float
+    expected_energy: float
+    prob: float
+    max_trunc: float
+    entropy: np.ndarray
+    max_bond: np.ndarray
+
+
+def mps_anneal(model: Ising, sched: Schedule, dt: float, chi: int, zz_substeps: int = 1,
+               bias: np.ndarray | None = None) -> AnnealResult:
+    """cuda/include/mps.hpp:76-82 through tessera_qa_anneal_ex."""
+    ei, ej, ew = _edges(model)
+    steps = sched.steps
+    spins = np.zeros(model.n, np.int8)
+    sc = np.zeros(4)
+    ent = np.zeros(max(steps, 1))
+    mb = np.zeros(max(steps, 1), np.int32)
+    a = np.ascontiguousarray(sched.a, np.float64)
+    b = np.ascontiguousarray(sched.b, np.float64)
+    bias_arr = None if bias is None else np.ascontiguousarray(bias, np.float64)
+    _check(lib().tessera_qa_anneal_ex(_p(model.h), model.n, _p(ei), _p(ej), _p(ew), model.m, _p(a), _p(b), steps,
+                                      dt, chi, zz_substeps, _p(bias_arr), _p(spins), _p(sc), _p(ent), _p(mb)))
+    return AnnealResult(spins, float(sc[0]), float(sc[1]), float(sc[2]), float(sc[3]), ent[:steps], mb[:steps])
+
+
+# ------------------------------------------------------------ quantum.rs mirror
+@dataclass
+class QaOutcome:
+    """rust/src/quantum.rs:27-43"""
+
+    spins: np.ndarray
+    energy: float
+    expected_energy: float
+    success_prob: float
+    entropy_trace: np.ndarray
+    energy_trace: np.ndarray
+
+
+def anneal_statevector(model: Ising, sched: Schedule, dt: float) -> QaOutcome:
+    """rust/src/quantum.rs:63-112 on the device (no n <= 28 cap)."""
+    if model.n < 1:
+        raise ValueError("need at least one spin")
+    ei, ej, ew = _edges(model)
+    steps = sched.steps
+    spins = np.zeros(model.n, np.int8)
+    prob = ctypes.c_double()
+    en = ctypes.c_double()
+    ex = ctypes.c_double()
+    ent = np.zeros(steps)
+    etr = np.zeros(steps)
+    a = np.ascontiguousarray(sched.a, np.float64)
+    b = np.ascontiguousarray(sched.b, np.float64)
+    _check(lib().tessera_sv_anneal(_p(model.h), model.n, _p(ei), _p(ej), _p(ew), model.m, _p(a), _p(b), steps, dt,
+                                   _p(spins), ctypes.addressof(prob), _p(ent), _p(etr), ctypes.addressof(en),
+                                   ctypes.addressof(ex)))
+    return QaOutcome(spins, en.value, ex.value, prob.value, ent, etr)
+
+
+def solve_qa(h, ei, ej, ew, steps: int, dt: float):
+    """rust/src/capi.rs:107-143 (smooth schedule). Returns (energy, spins, prob,
+    entropy, energy_trace)."""
+    h = np.ascontiguousarray(h, np.float64)
+    n = h.shape[0]
+    ei = None if ei is None else np.ascontiguousarray(ei, np.uintp)
+    ej = None if ej is None else np.ascontiguousarray(ej, np.uintp)
+    ew = None if ew is None else np.ascontiguousarray(ew, np.float64)
+    m = 0 if ew is None else ew.shape[0]
+    st = max(steps, 2)
+    spins = np.zeros(max(n, 1), np.int8)
+    prob = ctypes.c_double()
+    ent = np.zeros(st)
+    etr = np.zeros(st)
+    e = lib().tessera_solve_qa(_p(h), n, _p(ei), _p(ej), _p(ew), m, steps, dt, _p(spins), ctypes.addressof(prob),
+                               _p(ent), _p(etr))
+    return e, spins[:n], prob.value, ent, etr
+
+
+# ---------------------------------------------------------- native TEBD session
+class TebdSession:
+    """Device-resident MPS for repeated Trotter steps (tessera_tebd_* API)."""
+
+    def __init__(self, model: Ising, chi: int, dt: float, zz_substeps: int = 1):
+        ei, ej, ew = _edges(model)
+        h = np.ascontiguousarray(model.h, np.float64)
+        handle = ctypes.c_void_p()
+        _check(lib().tessera_tebd_create(_p(h), model.n, _p(ei), _p(ej), _p(ew), model.m, chi, dt, zz_substeps,
+                                         ctypes.byref(handle)))
+        self._h = handle
+        self.n = model.n
+        self.recorded = 0
+
+    def close(self):
+        if self._h:
+            lib().tessera_tebd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        _check(lib().tessera_tebd_reset(self._h))
+        self.recorded = 0
+
+    def randomize(self, seed: int, decay: float = 40.0):
+        _check(lib().tessera_tebd_randomize(self._h, seed, decay))
+        self.recorded = 0
+
+    def run(self, a, b, record: bool = True):
+        a = np.ascontiguousarray(a, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        _check(lib().tessera_tebd_run(self._h, _p(a), _p(b), a.shape[0], 1 if record else 0))
+        if record:
+            self.recorded += a.shape[0]
+
+    def sync(self):
+        ent = np.zeros(max(self.recorded, 1))
+        tr = ctypes.c_double()
+        bonds = np.zeros(self.n + 1, np.int32)
+        _check(lib().tessera_tebd_sync(self._h, _p(ent), self.recorded, ctypes.addressof(tr), _p(bonds)))
+        return ent[: self.recorded], tr.value, bonds
+
+    def readout(self):
+        spins = np.zeros(self.n, np.int8)
+        sc = np.zeros(4)
+        _check(lib().tessera_tebd_readout(self._h, _p(spins), _p(sc)))
+        return spins, sc
+
+    def dense(self) -> np.ndarray:
+        out = np.zeros(2 << self.n)
+        _check(lib().tessera_tebd_dense(self._h, _p(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    @property
+    def updates_per_step(self) -> int:
+        return int(lib().tessera_tebd_updates_per_step(self._h))

@@ -131,6 +131,16 @@ int tessera_tebd_readout(tessera_tebd* t, int8_t* out_s, double* out_scalars);
 /* Dense amplitudes (n <= 24): out[2*x], out[2*x+1] = re, im of psi(x), where bit
  * i of x is the physical index of site i (0 <-> spin +1). */
 int tessera_tebd_dense(tessera_tebd* t, double* out);
+/* Diagnostics since creation: SVDs run, Jacobi sweeps they took, out_rows[2] =
+ * column-pair visits and rotations weighted by column length (pair-rows; the
+ * one-sided Jacobi does 16 flops per visited pair-row and 18 more per rotated
+ * one), out_ms[2] = HIP-event milliseconds of the timed SVD launches and their
+ * count (see tessera_tebd_set_timing). Any output may be NULL. */
+int tessera_tebd_svd_stats(tessera_tebd* t, int64_t* out_calls, int64_t* out_sweeps, double* out_rows,
+                           double* out_ms);
+/* Record HIP events around every SVD launch from now on (on != 0); resets the
+ * accumulated time. */
+int tessera_tebd_set_timing(tessera_tebd* t, int on);
 /* Number of two-site updates one Trotter step performs for this model. */
 size_t tessera_tebd_updates_per_step(const tessera_tebd* t);
 void tessera_tebd_destroy(tessera_tebd* t);

@@ -210,7 +210,8 @@ static int hestenes_cols(Mat& b, Mat& v) {
                     gamma += std::conj(b.at(i, p)) * b.at(i, q);
                 }
                 const double g = std::abs(gamma);
-                if (g == 0.0 || g <= 1e-15 * std::sqrt(alpha * beta)) continue;
+                if (g == 0.0 || g <= (double)(m < 8 ? 8 : m) * 2.220446049250313e-16 * std::sqrt(alpha * beta))
+                    continue;
                 rotated = true;
                 const cplx ph = std::conj(gamma) / g;  // multiplies column q
                 const double zeta = (beta - alpha) / (2.0 * g);

@@ -304,6 +304,39 @@ int tessera_tebd_dense(tessera_tebd* t, double* out) {
     }
 }
 
+int tessera_tebd_svd_stats(tessera_tebd* t, int64_t* out_calls, int64_t* out_sweeps, double* out_rows,
+                           double* out_ms) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        const SvdStats st = t->eng->svd_stats();
+        if (out_calls) *out_calls = st.calls;
+        if (out_sweeps) *out_sweeps = st.sweeps;
+        if (out_rows) {
+            out_rows[0] = st.visited_rows;
+            out_rows[1] = st.rotated_rows;
+        }
+        if (out_ms) {
+            out_ms[0] = st.svd_ms;
+            out_ms[1] = (double)st.timed_launches;
+        }
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_set_timing(tessera_tebd* t, int on) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        t->eng->set_timing(on != 0);
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
 size_t tessera_tebd_updates_per_step(const tessera_tebd* t) { return t ? t->eng->updates_per_step() : 0; }
 
 void tessera_tebd_destroy(tessera_tebd* t) { delete t; }

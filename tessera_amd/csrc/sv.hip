@@ -16,15 +16,14 @@
 //   threads holds 16 amplitudes; rotations on register bits need no exchange,
 //   and LDS exchanges cycle local bits 0-3, 4-7, 8-11 into registers, so each
 //   amplitude sees its rotations in ascending qubit order as upstream does.
-// Every multiply and add is rounded separately (no FMA contraction) to follow
-// the upstream operation order; rotation cos/sin are computed on the host with
-// the same libm calls as upstream, the phase cos/sin per amplitude on the device.
+// Rotation cos/sin are computed on the host with the same libm calls as upstream;
+// phases come from per-tile tables (below), so results agree with upstream to
+// rounding (not bitwise).
 #include <cmath>
 
 #include "common.hpp"
 #include "sv.hpp"
 
-#pragma clang fp contract(off)
 
 namespace tq {
 
@@ -43,6 +42,21 @@ struct Model {
     const int* ej;
     const double* ew;
     const double* h;
+};
+
+// Energy terms of one pass, classified by where their bits sit in a tile:
+// L = local bits 0-5, H = local bits 6-11, B = bits fixed by the tile (base).
+// Per tile E(base|j) = C + T1[j & 63] + T2[j >> 6] - sum_{LH} w s s, where T1 holds
+// the LL, BL edges and L fields, T2 the HH, BH edges and H fields, C the BB
+// edges and B fields. Edge terms are (a, b, w); field terms (a, -, h).
+struct Term {
+    int a, b;
+    double w;
+};
+enum { kLL, kHH, kBL, kBH, kBB, kLH, kFL, kFH, kFB, kKinds };
+struct PassTerms {
+    const Term* t;
+    int off[kKinds + 1];
 };
 
 // quantum.rs:46-56 via ising.rs:23-32, bit set => +1.
@@ -158,25 +172,164 @@ __global__ void __launch_bounds__(1024) k_sv_resident(Model md, const double* sa
 }
 
 // ---------------------------------------------------------------------------
-// n >= 14: one streaming pass. `c` contiguous low bits, window starting at `lo`;
-// local bits set in `rotmask` are rotated. PHASE_IN applies phase(scale_in)
-// after the load; PHASE_OUT applies phase(scale_out) after the rotations and
-// accumulates the step observables into partials[block][6].
-template <bool PHASE_IN, bool PHASE_OUT>
-__global__ void __launch_bounds__(kThreads) k_sv_pass(double2* __restrict__ psi, Model md, int c, int lo,
-                                                      unsigned rotmask, double rc, double rs,
-                                                      double scale_in, double scale_out,
-                                                      long long ntiles, double* __restrict__ partials) {
-    __shared__ double2 sh[kTile + kTile / 16];
-    __shared__ double red[kThreads / 64][kObs];
+// n >= 14: streaming passes over 2^12-amplitude tiles.
+//
+// The X rotations of different qubits commute, so a step may rotate its bit
+// groups in any order; steps alternate ascending and descending group order so
+// that the last group of step k is the first group of step k+1. That shared
+// group is done in one pass: rotate(k), phase(k), observables(k), phase(k+1),
+// rotate(k+1). With g groups a step costs g-1 passes (2 at n = 30).
+//
+// Phases come from per-tile tables: bits of a tile split into L (local 0-5),
+// H (local 6-11) and B (fixed by the tile); E(base|j) = C + T1[j & 63] +
+// T2[j >> 6] - sum_{LH edges} w s s and exp(-i scale E) is the product of the
+// matching table phasors, so each amplitude costs a few complex products and no
+// transcendental.
+constexpr int kMaxTerms = 256;
+constexpr int kMaxCrossT = 64;
+
+struct PassArgs {
+    int c, lo;         // tile: 2^c contiguous amplitudes, window at lo (lo < 0: contiguous 4096 block)
+    unsigned mask;     // local bits of the rotation group
+    int rot_a, rot_b;  // rotate before / after the phase block
+    double ca, sa, cb, sb;
+    int phase_out, phase_in;
+    double scale_out, scale_in;
+    int best;          // track the argmax (last step)
+};
+
+__device__ inline double bit_spin(unsigned long long x, int b) { return ((x >> b) & 1ull) ? 1.0 : -1.0; }
+
+struct Best {
+    double p;
+    long long x;
+};
+__device__ inline Best better(Best a, Best b) {
+    if (b.p > a.p || (b.p == a.p && b.x < a.x)) return b;
+    return a;
+}
+
+// Per-tile energy tables (all 256 threads): tabE[0..127] = T1, T2; cst = C.
+__device__ void build_energy(const Term* tm, const int* off, unsigned long long base, double* tabE, double* cst) {
     const int t = threadIdx.x;
+    if (t < 128) {
+        const bool lowh = t < 64;
+        const unsigned u = (unsigned)(t & 63);
+        double v = 0.0;
+        const int kp = lowh ? kLL : kHH, kb = lowh ? kBL : kBH, kf = lowh ? kFL : kFH;
+        for (int q = off[kp]; q < off[kp + 1]; ++q) v -= tm[q].w * bit_spin(u, tm[q].a) * bit_spin(u, tm[q].b);
+        for (int q = off[kb]; q < off[kb + 1]; ++q) v -= tm[q].w * bit_spin(base, tm[q].a) * bit_spin(u, tm[q].b);
+        for (int q = off[kf]; q < off[kf + 1]; ++q) v -= tm[q].w * bit_spin(u, tm[q].a);
+        tabE[t] = v;
+    } else if (t < 192) {
+        const int lane = t & 63;
+        double v = 0.0;
+        for (int q = off[kBB] + lane; q < off[kBB + 1]; q += 64)
+            v -= tm[q].w * bit_spin(base, tm[q].a) * bit_spin(base, tm[q].b);
+        for (int q = off[kFB] + lane; q < off[kFB + 1]; q += 64) v -= tm[q].w * bit_spin(base, tm[q].a);
+        v = wave_sum(v);
+        if (lane == 0) *cst = v;
+    }
+}
+
+// Phasor tables exp(-i scale E) for the energy tables, the tile constant and
+// exp(+i scale w) of every cross edge (after build_energy + barrier).
+__device__ void build_phasors(const Term* tm, const int* off, double scale, const double* tabE, double cst,
+                              double2* tabP, double2* crossP, double2* cstP) {
+    const int t = threadIdx.x;
+    if (t < 128) {
+        double sn, cs;
+        sincos(scale * tabE[t], &sn, &cs);
+        tabP[t] = make_double2(cs, -sn);
+    } else if (t == 128) {
+        double sn, cs;
+        sincos(scale * cst, &sn, &cs);
+        *cstP = make_double2(cs, -sn);
+    } else if (t >= 192) {
+        for (int q = t - 192; q < off[kLH + 1] - off[kLH]; q += 64) {
+            double sn, cs;
+            sincos(scale * tm[off[kLH] + q].w, &sn, &cs);
+            crossP[q] = make_double2(cs, sn);  // factor when s_a s_b = +1
+        }
+    }
+}
+
+__device__ inline double tile_energy(const Term* tm, const int* off, int j, const double* tabE, double cst) {
+    double e = cst + tabE[j & 63] + tabE[64 + (j >> 6)];
+    for (int q = off[kLH]; q < off[kLH + 1]; ++q) {
+        const bool same = (((j >> tm[q].a) ^ (j >> tm[q].b)) & 1) == 0;
+        e += same ? -tm[q].w : tm[q].w;
+    }
+    return e;
+}
+
+__device__ inline double2 tile_phasor(const Term* tm, const int* off, int j, const double2* tabP, const double2* crossP,
+                                      double2 cstP) {
+    double2 f = cmul(cstP, cmul(tabP[j & 63], tabP[64 + (j >> 6)]));
+    for (int q = off[kLH]; q < off[kLH + 1]; ++q) {
+        const bool same = (((j >> tm[q].a) ^ (j >> tm[q].b)) & 1) == 0;
+        const double2 x = crossP[q - off[kLH]];
+        f = cmul(f, same ? x : cconj(x));
+    }
+    return f;
+}
+
+// Rotates the group bits of a tile held as v[r] in layout lay0: exchanges cycle
+// local bits 0-3, 4-7, 8-11 into registers; ends in lay0.
+__device__ inline void rotate_group(double2 (&v)[kRegs], double2* sh, unsigned mask, double c, double s) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) sh[sidx(lay0(t, r))] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay2(t, r))];
+    __syncthreads();
+    rotate_regs(v, mask & 15u, c, s);
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) sh[sidx(lay2(t, r))] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay1(t, r))];
+    __syncthreads();
+    rotate_regs(v, (mask >> 4) & 15u, c, s);
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) sh[sidx(lay1(t, r))] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay0(t, r))];
+    __syncthreads();
+    rotate_regs(v, (mask >> 8) & 15u, c, s);
+}
+
+template <bool PH>
+__global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ psi, PassTerms pt, PassArgs pa,
+                                                      long long ntiles, double* __restrict__ partials,
+                                                      double* __restrict__ best_p, long long* __restrict__ best_x) {
+    __shared__ double2 sh[kTile + kTile / 16];
+    __shared__ Term tm[kMaxTerms];
+    __shared__ int off[kKinds + 1];
+    __shared__ double tabE[128];
+    __shared__ double2 tabPo[128], tabPi[128];
+    __shared__ double2 crossPo[kMaxCrossT], crossPi[kMaxCrossT];
+    __shared__ double cst;
+    __shared__ double2 cstPo, cstPi;
+    __shared__ double red[kThreads / 64][kObs + 1];
+    __shared__ long long redx[kThreads / 64];
+    const int t = threadIdx.x;
+    if (PH) {
+        for (int q = t; q < pt.off[kKinds]; q += kThreads) tm[q] = pt.t[q];
+        if (t <= kKinds) off[t] = pt.off[t];
+        __syncthreads();
+    }
+    const int c = pa.c, lo = pa.lo;
     const int lo_span = lo - c;  // tile-id bits placed in [c, lo)
     const unsigned long long cmask = (1ull << c) - 1ull;
     double acc[kObs] = {0, 0, 0, 0, 0, 0};
+    Best best{-1.0, 0};
 
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         unsigned long long base;
-        if (lo_span <= 0) {
+        if (lo < 0) {
             base = (unsigned long long)tile << kTileBits;
         } else {
             const unsigned long long tl = (unsigned long long)tile & ((1ull << lo_span) - 1ull);
@@ -184,84 +337,91 @@ __global__ void __launch_bounds__(kThreads) k_sv_pass(double2* __restrict__ psi,
             base = (tl << c) | (th << (lo + (kTileBits - c)));
         }
         auto gidx = [&](int j) -> unsigned long long {
-            return base | ((unsigned long long)j & cmask) | (((unsigned long long)j >> c) << lo);
+            return lo < 0 ? (base | (unsigned long long)j)
+                          : (base | ((unsigned long long)j & cmask) | (((unsigned long long)j >> c) << lo));
         };
-        auto gidx_low = [&](int j) -> unsigned long long { return base | (unsigned long long)j; };
-
         double2 v[kRegs];
 #pragma unroll
-        for (int r = 0; r < kRegs; ++r) {
-            const int j = lay0(t, r);
-            v[r] = psi[lo_span < 0 ? gidx_low(j) : gidx(j)];
+        for (int r = 0; r < kRegs; ++r) v[r] = psi[gidx(lay0(t, r))];  // loads in flight during the tables
+        if (PH) {
+            build_energy(tm, off, base, tabE, &cst);
+            __syncthreads();
+            if (pa.phase_out) build_phasors(tm, off, pa.scale_out, tabE, cst, tabPo, crossPo, &cstPo);
+            if (pa.phase_in) build_phasors(tm, off, pa.scale_in, tabE, cst, tabPi, crossPi, &cstPi);
+            __syncthreads();
         }
-        if (PHASE_IN) {
+        // two rounds: [rotate(k)] [phase block] / [rotate(k+1)]; one copy of the rotation code
+#pragma unroll 1
+        for (int round = 0; round < 2; ++round) {
+        if (round == 0 ? pa.rot_a : pa.rot_b)
+            rotate_group(v, sh, pa.mask, round == 0 ? pa.ca : pa.cb, round == 0 ? pa.sa : pa.sb);
+        if (PH && round == 0) {
+            // phase block on the LDS copy of the tile (lay0 order), plain loops keep registers low
 #pragma unroll
-            for (int r = 0; r < kRegs; ++r) {
-                const int j = lay0(t, r);
-                const unsigned long long x = lo_span < 0 ? gidx_low(j) : gidx(j);
-                v[r] = apply_phase(v[r], scale_in, energy_of(md, x));
-            }
-        }
-        // ascending qubit order as quantum.rs:193: local bits 0-3, 4-7, 8-11
-#pragma unroll
-        for (int r = 0; r < kRegs; ++r) sh[sidx(lay0(t, r))] = v[r];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay2(t, r))];
-        __syncthreads();
-        rotate_regs(v, rotmask & 15u, rc, rs);
-#pragma unroll
-        for (int r = 0; r < kRegs; ++r) sh[sidx(lay2(t, r))] = v[r];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay1(t, r))];
-        __syncthreads();
-        rotate_regs(v, (rotmask >> 4) & 15u, rc, rs);
-#pragma unroll
-        for (int r = 0; r < kRegs; ++r) sh[sidx(lay1(t, r))] = v[r];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay0(t, r))];
-        __syncthreads();
-        rotate_regs(v, (rotmask >> 8) & 15u, rc, rs);
-        if (PHASE_OUT) {
-            const bool even = (t & 1) == 0;  // local bit 0 == global bit 0 == thread bit 0
-#pragma unroll
-            for (int r = 0; r < kRegs; ++r) {
-                const unsigned long long x = lo_span < 0 ? gidx_low(lay0(t, r)) : gidx(lay0(t, r));
-                const double e = energy_of(md, x);
-                v[r] = apply_phase(v[r], scale_out, e);
-                const double p = v[r].x * v[r].x + v[r].y * v[r].y;
-                acc[4] += p;
-                acc[5] += p * e;
-                const double2 w = make_double2(__shfl_xor(v[r].x, 1, 64), __shfl_xor(v[r].y, 1, 64));
-                if (even) {
-                    const double2 u = v[r];
-                    acc[0] += u.x * u.x + u.y * u.y;
-                    acc[1] += w.x * w.x + w.y * w.y;
-                    acc[2] += u.x * w.x + u.y * w.y;
-                    acc[3] += u.y * w.x - u.x * w.y;
+            for (int r = 0; r < kRegs; ++r) sh[sidx(lay0(t, r))] = v[r];
+            __syncthreads();
+            if (pa.phase_out) {
+#pragma unroll 1
+                for (int j = t; j < kTile; j += kThreads)
+                    sh[sidx(j)] = cmul(sh[sidx(j)], tile_phasor(tm, off, j, tabPo, crossPo, cstPo));
+                __syncthreads();
+#pragma unroll 1
+                for (int j = t; j < kTile; j += kThreads) {
+                    const double2 u = sh[sidx(j)];
+                    const double e = tile_energy(tm, off, j, tabE, cst);
+                    const double p = u.x * u.x + u.y * u.y;
+                    acc[4] += p;
+                    acc[5] += p * e;
+                    if (pa.best) best = better(best, Best{p, (long long)gidx(j)});
+                    if ((j & 1) == 0) {  // local bit 0 == global bit 0
+                        const double2 w = sh[sidx(j + 1)];
+                        acc[0] += u.x * u.x + u.y * u.y;
+                        acc[1] += w.x * w.x + w.y * w.y;
+                        acc[2] += u.x * w.x + u.y * w.y;
+                        acc[3] += u.y * w.x - u.x * w.y;
+                    }
                 }
+                __syncthreads();
             }
+            if (pa.phase_in) {
+#pragma unroll 1
+                for (int j = t; j < kTile; j += kThreads)
+                    sh[sidx(j)] = cmul(sh[sidx(j)], tile_phasor(tm, off, j, tabPi, crossPi, cstPi));
+                __syncthreads();
+            }
+#pragma unroll
+            for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay0(t, r))];
+            __syncthreads();
+        }
         }
 #pragma unroll
-        for (int r = 0; r < kRegs; ++r) {
-            const int j = lay0(t, r);
-            psi[lo_span < 0 ? gidx_low(j) : gidx(j)] = v[r];
-        }
+        for (int r = 0; r < kRegs; ++r) psi[gidx(lay0(t, r))] = v[r];
+        if (PH) __syncthreads();  // tables are rebuilt for the next tile
     }
-    if (PHASE_OUT) {
+    if (PH && pa.phase_out) {
 #pragma unroll
         for (int q = 0; q < kObs; ++q) acc[q] = wave_sum(acc[q]);
-        if ((t & 63) == 0)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) best = better(best, Best{__shfl_xor(best.p, o, 64), __shfl_xor(best.x, o, 64)});
+        if ((t & 63) == 0) {
             for (int q = 0; q < kObs; ++q) red[t >> 6][q] = acc[q];
+            red[t >> 6][kObs] = best.p;
+            redx[t >> 6] = best.x;
+        }
         __syncthreads();
-        if (t == 0)
+        if (t == 0) {
+            Best b{-1.0, 0};
             for (int q = 0; q < kObs; ++q) {
                 double s = 0.0;
                 for (int w = 0; w < kThreads / 64; ++w) s += red[w][q];
                 partials[(long long)blockIdx.x * kObs + q] = s;
             }
+            for (int w = 0; w < kThreads / 64; ++w) b = better(b, Best{red[w][kObs], redx[w]});
+            if (pa.best) {
+                best_p[blockIdx.x] = b.p;
+                best_x[blockIdx.x] = b.x;
+            }
+        }
     }
 }
 
@@ -290,16 +450,8 @@ __global__ void k_sv_fill(double2* psi, long long dim, double amp0) {
         psi[x] = make_double2(amp0, 0.0);
 }
 
-// Final measurement statistics (quantum.rs:89-102): total, sum p*E, and the
-// argmax with the lowest index among equal maxima.
-struct Best {
-    double p;
-    long long x;
-};
-__device__ inline Best better(Best a, Best b) {
-    if (b.p > a.p || (b.p == a.p && b.x < a.x)) return b;
-    return a;
-}
+// Final measurement statistics of the resident path (quantum.rs:89-102): total,
+// sum p*E, and the argmax with the lowest index among equal maxima.
 
 __global__ void __launch_bounds__(256) k_sv_measure(const double2* __restrict__ psi, Model md, long long dim,
                                                     double* __restrict__ part_sum, long long* __restrict__ part_idx) {
@@ -400,6 +552,8 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
     DevBuf<double> d_ent(pb.steps), d_en(pb.steps);
     hipStream_t st = nullptr;
 
+    double total = 0.0, expected = 0.0, best_p = -1.0;
+    long long best = 0;
     if (n <= kResidentMaxN) {
         DevBuf<double> d_a(pb.steps), d_b(pb.steps), d_c(pb.steps), d_s(pb.steps);
         TQ_HIP(hipMemcpy(d_b.p, pb.b.data(), pb.steps * sizeof(double), hipMemcpyHostToDevice));
@@ -408,70 +562,169 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
         hipLaunchKernelGGL(k_sv_resident, dim3(1), dim3(1024), 0, st, md, d_a.p, d_b.p, d_c.p, d_s.p, pb.dt,
                            pb.steps, amp0, d_ent.p, d_en.p, psi.p);
         TQ_LAUNCH_CHECK();
+        const int mblocks = (int)std::min<long long>((dim + 255) / 256, 2048);
+        DevBuf<double> ps(mblocks * 3);
+        DevBuf<long long> pi(mblocks);
+        hipLaunchKernelGGL(k_sv_measure, dim3(mblocks), dim3(256), 0, st, psi.p, md, dim, ps.p, pi.p);
+        TQ_LAUNCH_CHECK();
+        TQ_HIP(hipStreamSynchronize(st));
+        std::vector<double> hs(mblocks * 3);
+        std::vector<long long> hi(mblocks);
+        TQ_HIP(hipMemcpy(hs.data(), ps.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        TQ_HIP(hipMemcpy(hi.data(), pi.p, hi.size() * sizeof(long long), hipMemcpyDeviceToHost));
+        for (int b = 0; b < mblocks; ++b) {
+            total += hs[b * 3 + 0];
+            expected += hs[b * 3 + 1];
+            if (hs[b * 3 + 2] > best_p || (hs[b * 3 + 2] == best_p && hi[b] < best)) {
+                best_p = hs[b * 3 + 2];
+                best = hi[b];
+            }
+        }
     } else {
         const long long ntiles = dim >> kTileBits;
         const int nblocks = (int)std::min<long long>(ntiles, 4096);
         DevBuf<double> partials((std::size_t)nblocks * kObs);
+        DevBuf<double> bp(nblocks);
+        DevBuf<long long> bx(nblocks);
         hipLaunchKernelGGL(k_sv_fill, dim3(2048), dim3(256), 0, st, psi.p, dim, amp0);
         TQ_LAUNCH_CHECK();
-        // pass plan: low pass over bits [0,12), then windows of <= 9 bits
-        struct Pass {
+        // rotation groups: bits [0,12) as contiguous blocks, then windows of <= 9 bits
+        struct Group {
             int c, lo;
             unsigned mask;
         };
-        std::vector<Pass> plan;
-        plan.push_back({kTileBits, 0, (1u << kTileBits) - 1u});
+        std::vector<Group> groups;
+        groups.push_back({kTileBits, -1, (1u << kTileBits) - 1u});
         for (int lo = kTileBits; lo < n; lo += 9) {
             const int w = std::min(9, n - lo);
             const int c = kTileBits - w;
-            plan.push_back({c, lo, ((1u << w) - 1u) << c});
+            groups.push_back({c, lo, ((1u << w) - 1u) << c});
         }
-        for (int k = 0; k < pb.steps; ++k) {
-            const double scale = pb.b[k] * pb.dt * 0.5;
-            for (std::size_t p = 0; p < plan.size(); ++p) {
-                const Pass& ps = plan[p];
-                const bool first = p == 0, last = p + 1 == plan.size();
-                const int lo = first ? -1 : ps.lo;  // lo < c selects the contiguous-block addressing
-                const int c = first ? 0 : ps.c;
-                if (first)
-                    hipLaunchKernelGGL((k_sv_pass<true, false>), dim3(nblocks), dim3(kThreads), 0, st, psi.p, md,
-                                       c, lo, ps.mask, rc[k], rsn[k], scale, scale, ntiles, partials.p);
-                else if (last)
-                    hipLaunchKernelGGL((k_sv_pass<false, true>), dim3(nblocks), dim3(kThreads), 0, st, psi.p, md,
-                                       c, lo, ps.mask, rc[k], rsn[k], scale, scale, ntiles, partials.p);
-                else
-                    hipLaunchKernelGGL((k_sv_pass<false, false>), dim3(nblocks), dim3(kThreads), 0, st, psi.p, md,
-                                       c, lo, ps.mask, rc[k], rsn[k], scale, scale, ntiles, partials.p);
-                TQ_LAUNCH_CHECK();
+        const int ng = (int)groups.size();
+        // energy terms of every group, classified by tile position
+        std::vector<Term> terms;
+        std::vector<PassTerms> pts(ng);
+        std::vector<std::size_t> starts(ng);
+        for (int g = 0; g < ng; ++g) {
+            std::vector<int> role(n, -1);
+            for (int q = 0; q < kTileBits; ++q) {
+                const int gb = g == 0 ? q : (q < groups[g].c ? q : groups[g].lo + q - groups[g].c);
+                if (gb < n) role[gb] = q;
             }
-            hipLaunchKernelGGL(k_sv_finish_step, dim3(1), dim3(256), 0, st, partials.p, nblocks, n, k, d_ent.p,
-                               d_en.p);
+            std::vector<Term> bucket[kKinds];
+            auto cls = [](int q) { return q < 0 ? 0 : (q < 6 ? 1 : 2); };  // B, L, H
+            for (int k = 0; k < m; ++k) {
+                const int gi = pb.ei[k], gj = pb.ej[k];
+                const int qi = role[gi], qj = role[gj];
+                const int ci = cls(qi), cj = cls(qj);
+                const double w = pb.ew[k];
+                if (ci == 1 && cj == 1) bucket[kLL].push_back({qi, qj, w});
+                else if (ci == 2 && cj == 2) bucket[kHH].push_back({qi - 6, qj - 6, w});
+                else if (ci == 0 && cj == 0) bucket[kBB].push_back({gi, gj, w});
+                else if (ci == 0 && cj == 1) bucket[kBL].push_back({gi, qj, w});
+                else if (ci == 1 && cj == 0) bucket[kBL].push_back({gj, qi, w});
+                else if (ci == 0 && cj == 2) bucket[kBH].push_back({gi, qj - 6, w});
+                else if (ci == 2 && cj == 0) bucket[kBH].push_back({gj, qi - 6, w});
+                else bucket[kLH].push_back({qi, qj, w});
+            }
+            for (int i = 0; i < n; ++i) {
+                const int q = role[i];
+                if (cls(q) == 1) bucket[kFL].push_back({q, 0, pb.h[i]});
+                else if (cls(q) == 2) bucket[kFH].push_back({q - 6, 0, pb.h[i]});
+                else bucket[kFB].push_back({i, 0, pb.h[i]});
+            }
+            if (bucket[kLH].size() > (std::size_t)kMaxCrossT)
+                throw Error(-1, "too many couplings between the two halves of a tile");
+            starts[g] = terms.size();
+            int acc = 0;
+            for (int kd = 0; kd < kKinds; ++kd) {
+                pts[g].off[kd] = acc;
+                acc += (int)bucket[kd].size();
+                terms.insert(terms.end(), bucket[kd].begin(), bucket[kd].end());
+            }
+            pts[g].off[kKinds] = acc;
+            if (acc > kMaxTerms) throw Error(-1, "more than 256 energy terms (m + n) per tile");
+        }
+        DevBuf<Term> d_terms(std::max<std::size_t>(terms.size(), 1));
+        if (!terms.empty())
+            TQ_HIP(hipMemcpy(d_terms.p, terms.data(), terms.size() * sizeof(Term), hipMemcpyHostToDevice));
+        for (int g = 0; g < ng; ++g) pts[g].t = d_terms.p + starts[g];
+
+        auto launch = [&](int g, const PassArgs& pa, bool ph) {
+            if (ph)
+                hipLaunchKernelGGL((k_sv_pass<true>), dim3(nblocks), dim3(kThreads), 0, st, psi.p, pts[g], pa, ntiles,
+                                   partials.p, bp.p, bx.p);
+            else
+                hipLaunchKernelGGL((k_sv_pass<false>), dim3(nblocks), dim3(kThreads), 0, st, psi.p, pts[g], pa,
+                                   ntiles, partials.p, bp.p, bx.p);
             TQ_LAUNCH_CHECK();
+        };
+        auto base_args = [&](int g) {
+            PassArgs pa{};
+            pa.c = g == 0 ? 0 : groups[g].c;
+            pa.lo = groups[g].lo;
+            pa.mask = groups[g].mask;
+            return pa;
+        };
+        for (int k = 0; k < pb.steps; ++k) {
+            // ascending group order on even steps, descending on odd steps
+            auto grp = [&](int step, int idx) { return (step % 2 == 0) ? idx : ng - 1 - idx; };
+            for (int idx = 0; idx < ng; ++idx) {
+                const int g = grp(k, idx);
+                if (idx == 0 && k > 0) continue;  // rotated at the end of the previous step
+                PassArgs pa = base_args(g);
+                if (idx == 0) {  // very first pass: phase(0), rotate(0)
+                    pa.phase_in = 1;
+                    pa.scale_in = pb.b[0] * pb.dt * 0.5;
+                    pa.rot_b = 1;
+                    pa.cb = rc[0];
+                    pa.sb = rsn[0];
+                    launch(g, pa, true);
+                } else if (idx < ng - 1) {
+                    pa.rot_a = 1;
+                    pa.ca = rc[k];
+                    pa.sa = rsn[k];
+                    launch(g, pa, false);
+                } else {
+                    pa.rot_a = 1;
+                    pa.ca = rc[k];
+                    pa.sa = rsn[k];
+                    pa.phase_out = 1;
+                    pa.scale_out = pb.b[k] * pb.dt * 0.5;
+                    if (k + 1 < pb.steps) {
+                        pa.phase_in = 1;
+                        pa.scale_in = pb.b[k + 1] * pb.dt * 0.5;
+                        pa.rot_b = 1;
+                        pa.cb = rc[k + 1];
+                        pa.sb = rsn[k + 1];
+                    } else {
+                        pa.best = 1;
+                    }
+                    launch(g, pa, true);
+                    hipLaunchKernelGGL(k_sv_finish_step, dim3(1), dim3(256), 0, st, partials.p, nblocks, n, k,
+                                       d_ent.p, d_en.p);
+                    TQ_LAUNCH_CHECK();
+                }
+            }
+        }
+        TQ_HIP(hipStreamSynchronize(st));
+        std::vector<double> hs((std::size_t)nblocks * kObs), hbp(nblocks);
+        std::vector<long long> hbx(nblocks);
+        TQ_HIP(hipMemcpy(hs.data(), partials.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        TQ_HIP(hipMemcpy(hbp.data(), bp.p, nblocks * sizeof(double), hipMemcpyDeviceToHost));
+        TQ_HIP(hipMemcpy(hbx.data(), bx.p, nblocks * sizeof(long long), hipMemcpyDeviceToHost));
+        for (int b = 0; b < nblocks; ++b) {
+            total += hs[(std::size_t)b * kObs + 4];
+            expected += hs[(std::size_t)b * kObs + 5];
+            if (hbp[b] > best_p || (hbp[b] == best_p && hbx[b] < best)) {
+                best_p = hbp[b];
+                best = hbx[b];
+            }
         }
     }
-
-    const int mblocks = (int)std::min<long long>((dim + 255) / 256, 2048);
-    DevBuf<double> ps(mblocks * 3);
-    DevBuf<long long> pi(mblocks);
-    hipLaunchKernelGGL(k_sv_measure, dim3(mblocks), dim3(256), 0, st, psi.p, md, dim, ps.p, pi.p);
-    TQ_LAUNCH_CHECK();
     TQ_HIP(hipStreamSynchronize(st));
     TQ_HIP(hipMemcpy(out.entropy.data(), d_ent.p, pb.steps * sizeof(double), hipMemcpyDeviceToHost));
     TQ_HIP(hipMemcpy(out.energy_trace.data(), d_en.p, pb.steps * sizeof(double), hipMemcpyDeviceToHost));
-    std::vector<double> hs(mblocks * 3);
-    std::vector<long long> hi(mblocks);
-    TQ_HIP(hipMemcpy(hs.data(), ps.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
-    TQ_HIP(hipMemcpy(hi.data(), pi.p, hi.size() * sizeof(long long), hipMemcpyDeviceToHost));
-    double total = 0.0, expected = 0.0, best_p = -1.0;
-    long long best = 0;
-    for (int b = 0; b < mblocks; ++b) {
-        total += hs[b * 3 + 0];
-        expected += hs[b * 3 + 1];
-        if (hs[b * 3 + 2] > best_p || (hs[b * 3 + 2] == best_p && hi[b] < best)) {
-            best_p = hs[b * 3 + 2];
-            best = hi[b];
-        }
-    }
     out.spins.resize(n);
     for (int i = 0; i < n; ++i) out.spins[i] = ((best >> i) & 1) ? 1 : -1;
     // energy of the argmax in the upstream order (quantum.rs:106)

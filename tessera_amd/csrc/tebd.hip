@@ -21,7 +21,10 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
+
+#include <hip/hip_cooperative_groups.h>
 
 #include "tebd.hpp"
 
@@ -31,7 +34,9 @@ namespace {
 
 constexpr int kSvdThreads = 1024;
 constexpr int kMaxCols = 512;  // 2 * chi for chi <= 256
-constexpr double kTol = 1e-15;
+// Convergence: a column pair is orthogonal when |x^H y| <= m * eps * |x| |y|,
+// the level at which FP64 dot products of length m stop being informative.
+__device__ inline double jacobi_tol(int m) { return (double)(m < 8 ? 8 : m) * 2.220446049250313e-16; }
 
 __device__ inline double2 ld_site(const double2* sites, std::size_t off, int stride, int a, int s, int b) {
     return sites[off + ((std::size_t)a * 2 + s) * stride + b];
@@ -122,7 +127,7 @@ __device__ inline void rr_pair(int r, int k, int cp, int& i, int& j) {
 
 __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, int ld, int* dims, int p, int chi,
                                                      double* __restrict__ sv_out, int* __restrict__ perm_out,
-                                                     double* trunc, int use_lds) {
+                                                     double* trunc, int use_lds, unsigned long long* stats) {
     extern __shared__ double2 lds[];
     __shared__ double s_nrm[kMaxCols];
     __shared__ int s_perm[kMaxCols];
@@ -144,7 +149,10 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
         __syncthreads();
     }
     const int cp = c + (c & 1);
+    int sweeps_done = 0;
+    unsigned long long visited = 0, rotated = 0;  // pair visits / rotations (wave-local, lane 0 counts)
     for (int sweep = 0; sweep < 60 && cp >= 2; ++sweep) {
+        ++sweeps_done;
         if (tid == 0) s_flag = 0;
         __syncthreads();
         for (int r = 0; r < cp - 1; ++r) {
@@ -167,7 +175,9 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
                 gr = wave_sum(gr);
                 gi = wave_sum(gi);
                 const double gabs = sqrt(gr * gr + gi * gi);
-                if (!(gabs > 0.0) || gabs <= kTol * sqrt(al * be)) continue;
+                ++visited;
+                if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * sqrt(al * be)) continue;
+                ++rotated;
                 if (lane == 0) s_flag = 1;
                 const double2 ph = make_double2(gr / gabs, -gi / gabs);  // conj(gamma)/|gamma|
                 const double zeta = (be - al) / (2.0 * gabs);
@@ -222,6 +232,12 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
             if (*trunc < tr) *trunc = tr;
         }
         dims[p + 1] = keep;
+        stats[0] += 1;
+        stats[1] += sweeps_done;
+    }
+    if (lane == 0) {
+        atomicAdd(&stats[2], visited * (unsigned long long)m);
+        atomicAdd(&stats[3], rotated * (unsigned long long)m);
     }
     for (int q = tid; q < c; q += blockDim.x) {
         sv_out[q] = s_nrm[s_perm[q]];
@@ -232,6 +248,180 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
             const int col = idx / m, row = idx % m;
             Wg[(std::size_t)col * ld + row] = lds[idx];
         }
+    }
+}
+
+// ---- one-sided block Jacobi SVD over a cooperative grid ---------------------
+// Columns are split into blocks of 8; each workgroup holds two blocks in LDS and
+// the blocks meet in round-robin tournament order, one grid barrier per outer
+// step. At the first outer step of a sweep every block also rotates its own
+// column pairs, so each sweep visits every column pair exactly once. Rotation
+// formulas, FP64 angles and the convergence test are those of k_svd.
+constexpr int kBlk = 8;
+constexpr int kDistThreads = 512;  // 8 waves: one column pair per wave
+
+__device__ inline void jacobi_pair_lds(double2* x, double2* y, int m, int lane, bool& rotated,
+                                       unsigned long long& nvis, unsigned long long& nrot) {
+    double al = 0, be = 0, gr = 0, gi = 0;
+    for (int row = lane; row < m; row += 64) {
+        const double2 u = x[row], v = y[row];
+        al += u.x * u.x + u.y * u.y;
+        be += v.x * v.x + v.y * v.y;
+        gr += u.x * v.x + u.y * v.y;
+        gi += u.x * v.y - u.y * v.x;
+    }
+    al = wave_sum(al);
+    be = wave_sum(be);
+    gr = wave_sum(gr);
+    gi = wave_sum(gi);
+    const double gabs = sqrt(gr * gr + gi * gi);
+    ++nvis;
+    if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * sqrt(al * be)) return;
+    ++nrot;
+    rotated = true;
+    const double2 ph = make_double2(gr / gabs, -gi / gabs);
+    const double zeta = (be - al) / (2.0 * gabs);
+    const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+    for (int row = lane; row < m; row += 64) {
+        const double2 u = x[row], v = cmul(y[row], ph);
+        x[row] = make_double2(cs * u.x - sn * v.x, cs * u.y - sn * v.y);
+        y[row] = make_double2(sn * u.x + cs * v.x, sn * u.y + cs * v.y);
+    }
+}
+
+__global__ void __launch_bounds__(kDistThreads) k_svd_dist(double2* __restrict__ Wg, int ld, int* dims, int p,
+                                                           int chi, double* __restrict__ sv_out,
+                                                           int* __restrict__ perm_out, double* trunc, int* flags,
+                                                           unsigned long long* stats) {
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    extern __shared__ double2 lds[];
+    __shared__ double s_nrm[kMaxCols];
+    __shared__ int s_perm[kMaxCols];
+    __shared__ int s_rot;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int Dl = dims[p], Dr = dims[p + 2];
+    const int R = 2 * Dl, C = 2 * Dr;
+    const bool tall = C <= R;
+    const int m = tall ? R : C, c = tall ? C : R;
+    const int nb = (c + kBlk - 1) / kBlk;
+    const int nbp = nb + (nb & 1) < 2 ? 2 : nb + (nb & 1);
+    const int P = nbp / 2;
+    const int me = blockIdx.x;
+    if (me == 0 && tid == 0) {
+        flags[0] = 0;
+        flags[1] = 0;
+    }
+    grid.sync();
+    int sweeps_done = 0;
+    unsigned long long nvis = 0, nrot = 0;
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        ++sweeps_done;
+        for (int r = 0; r < nbp - 1; ++r) {
+            if (me < P) {
+                int bi, bj;
+                rr_pair(r, me, nbp, bi, bj);
+                if (tid == 0) s_rot = 0;
+                // load the two blocks: LDS column q <-> block (q < 8 ? bi : bj), member q & 7
+                for (int q = wave; q < 2 * kBlk; q += kDistThreads / 64) {
+                    const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
+                    double2* dst = lds + (std::size_t)q * m;
+                    if (col < c) {
+                        const double2* src = Wg + (std::size_t)col * ld;
+                        for (int row = lane; row < m; row += 64) dst[row] = src[row];
+                    }
+                }
+                __syncthreads();
+                bool rotated = false;
+                if (r == 0) {  // pairs inside each block, once per sweep
+                    for (int s = 0; s < kBlk - 1; ++s) {
+                        const int half = wave >> 2, k = wave & 3;  // waves 0-3 block bi, 4-7 block bj
+                        int i, j;
+                        rr_pair(s, k, kBlk, i, j);
+                        const int blk = half ? bj : bi;
+                        if (blk * kBlk + i < c && blk * kBlk + j < c)
+                            jacobi_pair_lds(lds + (std::size_t)(half * kBlk + i) * m,
+                                            lds + (std::size_t)(half * kBlk + j) * m, m, lane, rotated, nvis, nrot);
+                        __syncthreads();
+                    }
+                }
+                for (int s = 0; s < kBlk; ++s) {  // cross pairs (bi member w, bj member w+s)
+                    const int i = wave, j = (wave + s) & (kBlk - 1);
+                    if (bi * kBlk + i < c && bj * kBlk + j < c)
+                        jacobi_pair_lds(lds + (std::size_t)i * m, lds + (std::size_t)(kBlk + j) * m, m, lane,
+                                        rotated, nvis, nrot);
+                    __syncthreads();
+                }
+                if (rotated && lane == 0) s_rot = 1;
+                __syncthreads();
+                if (tid == 0 && s_rot) atomicOr(&flags[sweep & 1], 1);
+                for (int q = wave; q < 2 * kBlk; q += kDistThreads / 64) {
+                    const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
+                    const double2* src = lds + (std::size_t)q * m;
+                    if (col < c) {
+                        double2* dst = Wg + (std::size_t)col * ld;
+                        for (int row = lane; row < m; row += 64) dst[row] = src[row];
+                    }
+                }
+            }
+            grid.sync();
+        }
+        const int f = __hip_atomic_load(&flags[sweep & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        grid.sync();
+        if (me == 0 && tid == 0) flags[sweep & 1] = 0;
+        if (!f) break;
+    }
+    if (lane == 0 && nvis) {
+        atomicAdd(&stats[2], nvis * (unsigned long long)m);
+        atomicAdd(&stats[3], nrot * (unsigned long long)m);
+    }
+    // column norms -> sv_out (scratch), then workgroup 0 sorts and truncates
+    for (int col = me * (kDistThreads / 64) + wave; col < c; col += gridDim.x * (kDistThreads / 64)) {
+        const double2* x = Wg + (std::size_t)col * ld;
+        double s = 0.0;
+        for (int row = lane; row < m; row += 64) s += x[row].x * x[row].x + x[row].y * x[row].y;
+        s = wave_sum(s);
+        if (lane == 0) sv_out[col] = sqrt(s);
+    }
+    grid.sync();
+    if (me != 0) return;
+    for (int q = tid; q < c; q += blockDim.x) s_nrm[q] = sv_out[q];
+    __syncthreads();
+    for (int q = tid; q < c; q += blockDim.x) {
+        const double v = s_nrm[q];
+        int rank = 0;
+        for (int o = 0; o < c; ++o) {
+            const double u = s_nrm[o];
+            rank += (u > v) || (u == v && o < q);
+        }
+        s_perm[rank] = q;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double total = 0.0;
+        for (int q = 0; q < c; ++q) {
+            const double s = s_nrm[s_perm[q]];
+            total += s * s;
+        }
+        int keep = min(chi, c);
+        while (keep > 1 && s_nrm[s_perm[keep - 1]] < 1e-14) --keep;
+        double disc = 0.0;
+        for (int q = keep; q < c; ++q) {
+            const double s = s_nrm[s_perm[q]];
+            disc += s * s;
+        }
+        if (total > 0) {
+            const double tr = disc / total;
+            if (*trunc < tr) *trunc = tr;
+        }
+        dims[p + 1] = keep;
+        stats[0] += 1;
+        stats[1] += sweeps_done;
+    }
+    for (int q = tid; q < c; q += blockDim.x) {
+        sv_out[q] = s_nrm[s_perm[q]];
+        perm_out[q] = s_perm[q];
     }
 }
 
@@ -500,6 +690,10 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     for (int c : cap_) maxcap = std::max(maxcap, c);
     maxcap_ = maxcap;
     const int ld = 2 * maxcap;
+    ld_ = ld;
+    flags_.alloc(2);
+    stats_.alloc(4);
+    TQ_HIP(hipMemset(stats_.p, 0, 4 * sizeof(unsigned long long)));
     mat_.alloc((std::size_t)ld * ld);
     work_.alloc((std::size_t)ld * ld);
     sv_.alloc(ld);
@@ -520,11 +714,18 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     TQ_HIP(hipMemcpy(caps_dev_.p, cap_.data(), (n_ + 1) * sizeof(int), hipMemcpyHostToDevice));
     TQ_HIP(hipMemcpy(off_dev_.p, off_.data(), (n_ + 1) * sizeof(std::size_t), hipMemcpyHostToDevice));
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
+    TQ_HIP(hipFuncSetAttribute((const void*)k_svd_dist, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
+    const char* fd = std::getenv("TESSERA_FORCE_DIST_SVD");  // test knob: distributed SVD at every size
+    force_dist_ = fd && fd[0] == '1';
     build_plan();
     reset_product();
 }
 
 Tebd::~Tebd() {
+    for (auto& e : events_) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -582,18 +783,41 @@ void Tebd::randomize(uint64_t seed, double decay) {
 }
 
 void Tebd::two_site(int p, const Gate4& g) {
-    const int ld = (int)std::lround(std::sqrt((double)mat_.n));
+    const int ld = ld_;
     const int Dlc = cap_[p], Drc = cap_[p + 2];
     dim3 tgrid((Drc + 15) / 16, (Dlc + 15) / 16);
     hipLaunchKernelGGL(k_theta, tgrid, dim3(16, 16), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p, p, g,
                        mat_.p, work_.p, ld);
     TQ_LAUNCH_CHECK();
-    // worst-case matrix at this bond decides LDS residency (both dims <= 2*cap)
+    // worst-case matrix at this bond decides the kernel (both dims <= 2*cap)
     const std::size_t bytes = (std::size_t)(2 * Dlc) * (2 * Drc) * sizeof(double2);
-    const int use_lds = bytes <= 144 * 1024 ? 1 : 0;
-    hipLaunchKernelGGL(k_svd, dim3(1), dim3(kSvdThreads), use_lds ? bytes : 0, stream_, work_.p, ld, dims_.p, p,
-                       chi_, sv_.p, perm_.p, trunc_.p, use_lds);
-    TQ_LAUNCH_CHECK();
+    if (timing_) {
+        if (ev_next_ >= events_.size()) {
+            hipEvent_t a, b;
+            TQ_HIP(hipEventCreate(&a));
+            TQ_HIP(hipEventCreate(&b));
+            events_.push_back({a, b});
+        }
+        TQ_HIP(hipEventRecord(events_[ev_next_].first, stream_));
+    }
+    if (bytes <= 144 * 1024 && !force_dist_) {
+        hipLaunchKernelGGL(k_svd, dim3(1), dim3(kSvdThreads), bytes, stream_, work_.p, ld, dims_.p, p, chi_, sv_.p,
+                           perm_.p, trunc_.p, 1, stats_.p);
+        TQ_LAUNCH_CHECK();
+    } else {
+        const int ccap = std::min(2 * Dlc, 2 * Drc), mcap = std::max(2 * Dlc, 2 * Drc);
+        const int nb = (ccap + kBlk - 1) / kBlk;
+        const int nbp = std::max(2, nb + (nb & 1));
+        const unsigned nwg = (unsigned)(nbp / 2);
+        const std::size_t shm = (std::size_t)2 * kBlk * mcap * sizeof(double2);
+        int* flags = flags_.p;
+        int pp = p, cc = chi_;
+        unsigned long long* stats = stats_.p;
+        void* args[] = {(void*)&work_.p, (void*)&ld_,     (void*)&dims_.p,  (void*)&pp,    (void*)&cc,
+                        (void*)&sv_.p,   (void*)&perm_.p, (void*)&trunc_.p, (void*)&flags, (void*)&stats};
+        TQ_HIP(hipLaunchCooperativeKernel((const void*)k_svd_dist, dim3(nwg), dim3(kDistThreads), args, shm, stream_));
+    }
+    if (timing_) TQ_HIP(hipEventRecord(events_[ev_next_++].second, stream_));
     const long long work = (long long)(2 * Dlc) * std::min(chi_, 2 * Drc) + (long long)std::min(chi_, 2 * Dlc) * (2 * Drc);
     hipLaunchKernelGGL(k_recon, dim3((unsigned)grid_for(work, 256)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
                        caps_dev_.p, dims_.p, p, mat_.p, work_.p, ld, sv_.p, perm_.p);
@@ -695,7 +919,23 @@ void Tebd::entropy_sweep(int rec) {
     TQ_LAUNCH_CHECK();
 }
 
-void Tebd::sync() { TQ_HIP(hipStreamSynchronize(stream_)); }
+void Tebd::sync() {
+    TQ_HIP(hipStreamSynchronize(stream_));
+    for (std::size_t q = 0; q < ev_next_; ++q) {  // fold recorded SVD launch times
+        float ms = 0.f;
+        TQ_HIP(hipEventElapsedTime(&ms, events_[q].first, events_[q].second));
+        timed_ms_ += ms;
+        ++timed_n_;
+    }
+    ev_next_ = 0;
+}
+
+void Tebd::set_timing(bool on) {
+    sync();
+    timing_ = on;
+    timed_ms_ = 0.0;
+    timed_n_ = 0;
+}
 
 std::vector<double> Tebd::entropy(int count) {
     std::vector<double> out(count, 0.0);
@@ -703,6 +943,20 @@ std::vector<double> Tebd::entropy(int count) {
         TQ_HIP(hipMemcpyAsync(out.data(), trace_.p, count * sizeof(double), hipMemcpyDeviceToHost, stream_));
     sync();
     return out;
+}
+
+SvdStats Tebd::svd_stats() {
+    unsigned long long h[4] = {0, 0, 0, 0};
+    TQ_HIP(hipMemcpyAsync(h, stats_.p, sizeof(h), hipMemcpyDeviceToHost, stream_));
+    sync();
+    SvdStats st;
+    st.calls = (long long)h[0];
+    st.sweeps = (long long)h[1];
+    st.visited_rows = (double)h[2];
+    st.rotated_rows = (double)h[3];
+    st.svd_ms = timed_ms_;
+    st.timed_launches = timed_n_;
+    return st;
 }
 
 double Tebd::max_trunc() {

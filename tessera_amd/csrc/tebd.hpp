@@ -18,6 +18,16 @@ struct TebdModel {
     std::vector<double> bias;  // empty or n entries
 };
 
+// Diagnostics of the SVD kernels: calls, Jacobi sweeps, column-pair visits and
+// rotations weighted by the column length (pair-rows), and the HIP-event time of
+// the SVD launches while timing is on.
+struct SvdStats {
+    long long calls = 0, sweeps = 0;
+    double visited_rows = 0, rotated_rows = 0;
+    double svd_ms = 0;
+    long long timed_launches = 0;
+};
+
 // One two-site update of the routed ZZ block.
 struct Update {
     int site;   // acts on (site, site+1)
@@ -38,6 +48,8 @@ public:
     void sync();
     std::vector<double> entropy(int count);      // recorded slots [0, count)
     double max_trunc();
+    SvdStats svd_stats();
+    void set_timing(bool on);  // HIP events around every SVD launch
     std::vector<int> bonds();                    // n+1 bond dims
     struct Readout {
         std::vector<int8_t> spins;
@@ -79,6 +91,15 @@ private:
     DevBuf<double> sv_;            // sorted singular values
     DevBuf<int> perm_;             // sorted position -> column
     DevBuf<double> trunc_;         // running max discarded weight
+    DevBuf<int> flags_;            // convergence flags of the distributed SVD
+    DevBuf<unsigned long long> stats_;  // calls, sweeps, visited pair-rows, rotated pair-rows
+    bool timing_ = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
+    std::size_t ev_next_ = 0;
+    double timed_ms_ = 0;
+    long long timed_n_ = 0;
+    int ld_ = 0;                   // leading dimension of mat_/work_
+    bool force_dist_ = false;
     DevBuf<double> h_, bias_;
     DevBuf<double2> envA_, envB_, envT_;
     DevBuf<double> trace_;

@@ -73,6 +73,10 @@ def lib() -> ctypes.CDLL:
         L.tessera_tebd_readout.argtypes = [vp, vp, vp]
         L.tessera_tebd_dense.restype = ctypes.c_int
         L.tessera_tebd_dense.argtypes = [vp, vp]
+        L.tessera_tebd_svd_stats.restype = ctypes.c_int
+        L.tessera_tebd_svd_stats.argtypes = [vp, vp, vp, vp, vp]
+        L.tessera_tebd_set_timing.restype = ctypes.c_int
+        L.tessera_tebd_set_timing.argtypes = [vp, ctypes.c_int]
         L.tessera_tebd_updates_per_step.restype = _usz
         L.tessera_tebd_updates_per_step.argtypes = [vp]
         L.tessera_tebd_destroy.restype = None
@@ -279,6 +283,20 @@ class TebdSession:
         out = np.zeros(2 << self.n)
         _check(lib().tessera_tebd_dense(self._h, _p(out)))
         return out[0::2] + 1j * out[1::2]
+
+    def svd_stats(self) -> dict:
+        """SVDs run, Jacobi sweeps, visited/rotated pair-rows and timed SVD ms since creation."""
+        calls = ctypes.c_int64()
+        sweeps = ctypes.c_int64()
+        rows = np.zeros(2)
+        ms = np.zeros(2)
+        _check(lib().tessera_tebd_svd_stats(self._h, ctypes.addressof(calls), ctypes.addressof(sweeps), _p(rows),
+                                            _p(ms)))
+        return dict(calls=calls.value, sweeps=sweeps.value, visited_rows=rows[0], rotated_rows=rows[1],
+                    svd_ms=ms[0], timed_launches=int(ms[1]))
+
+    def set_timing(self, on: bool = True):
+        _check(lib().tessera_tebd_set_timing(self._h, 1 if on else 0))
 
     @property
     def updates_per_step(self) -> int:

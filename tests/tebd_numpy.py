@@ -1,0 +1,56 @@
+"""Independent dense-linear-algebra TEBD (numpy/LAPACK SVD) of the upstream
+algorithm (cuda/src/mps.cpp:41-145, 295-356): a third implementation used to
+measure how well-conditioned a truncated run is."""
+import math
+
+import numpy as np
+
+
+def tebd_numpy(model, sched, dt, chi):
+    n = model.n
+    sites = [np.zeros((1, 2, 1), complex) for _ in range(n)]
+    for t in sites:
+        t[0, 0, 0] = t[0, 1, 0] = 1 / math.sqrt(2)
+    trunc = 0.0
+    bonds = []
+    swap = np.zeros((4, 4))
+    swap[0, 0] = swap[1, 2] = swap[2, 1] = swap[3, 3] = 1
+
+    def gate2(p, g):
+        nonlocal trunc
+        a, c = sites[p], sites[p + 1]
+        dl, dr = a.shape[0], c.shape[2]
+        th = np.einsum("asc,ctb->astb", a, c).reshape(dl, 4, dr)
+        th = np.einsum("rs,asb->arb", g, th).reshape(dl * 2, 2 * dr)
+        u, s, vh = np.linalg.svd(th, full_matrices=False)
+        keep = min(chi, len(s))
+        while keep > 1 and s[keep - 1] < 1e-14:
+            keep -= 1
+        tot = (s ** 2).sum()
+        if tot > 0:
+            trunc = max(trunc, (s[keep:] ** 2).sum() / tot)
+        sites[p] = u[:, :keep].reshape(dl, 2, keep)
+        sites[p + 1] = (s[:keep, None] * vh[:keep]).reshape(keep, 2, dr)
+
+    def layer(A, B):
+        for i in range(n):
+            g = np.array([[-B * model.h[i], -A], [-A, B * model.h[i]]])
+            w, v = np.linalg.eigh(g)
+            u1 = v @ np.diag(np.exp(-1j * dt / 2 * w)) @ v.conj().T
+            sites[i] = np.einsum("st,atb->asb", u1, sites[i])
+
+    for A, B in zip(sched.a, sched.b):
+        layer(A, B)
+        for i, j, w in zip(model.ei, model.ej, model.ew):
+            i, j = sorted((int(i), int(j)))
+            if i == j:
+                continue
+            zz = np.diag(np.exp(-1j * (-B * w * dt) * np.array([1, -1, -1, 1])))
+            for k in range(j, i + 1, -1):
+                gate2(k - 1, swap)
+            gate2(i, zz)
+            for k in range(i + 1, j):
+                gate2(k, swap)
+        layer(A, B)
+        bonds.append(max(t.shape[2] for t in sites))
+    return trunc, bonds

@@ -1,0 +1,98 @@
+"""The C-ABI library without a GPU: it loads, exports every symbol
+include/tessera_qa.h declares, and rejects bad arguments before any device work
+(upstream semantics for tessera_solve_qa: 0.0 on null pointers, NaN for n > 24,
+rust/src/capi.rs:121-126)."""
+import math
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tessera_amd import instances as I
+from tessera_amd import qa
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "tessera_qa.h"
+
+
+def declared_symbols():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(tessera_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(qa.library_path())], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (tessera_\w+)", out))
+    missing = [s for s in declared_symbols() if s not in exported]
+    assert not missing, missing
+    assert "tessera_qa_anneal" in exported  # cuda/src/ffi.cpp:10 drop-in
+
+
+def test_library_has_gfx950_code_object():
+    data = qa.library_path().read_bytes()
+    assert b"gfx950" in data
+
+
+def _args(h, ei, ej, ew, a, b):
+    P = qa._p
+    return P(h), h.shape[0], P(ei), P(ej), P(ew), ew.shape[0], P(a), P(b), a.shape[0]
+
+
+def test_extended_entry_rejects_bad_arguments():
+    L = qa.lib()
+    h = np.zeros(3)
+    ew = np.ones(2)
+    a = np.ones(4)
+    b = np.ones(4)
+    sp = np.zeros(3, np.int8)
+    sc = np.zeros(4)
+    bad_idx = np.array([0, 5], np.uintp)
+    ok = np.array([1, 2], np.uintp)
+    rc = L.tessera_qa_anneal_ex(*_args(h, bad_idx, ok, ew, a, b), 0.1, 8, 1, None, qa._p(sp), qa._p(sc), None, None)
+    assert rc == -1 and b"edge index" in L.tessera_last_error()
+    hn = np.array([0.0, math.nan, 0.0])
+    rc = L.tessera_qa_anneal_ex(*_args(hn, ok - 1, ok, ew, a, b), 0.1, 8, 1, None, qa._p(sp), qa._p(sc), None, None)
+    assert rc == -4
+    rc = L.tessera_qa_anneal_ex(*_args(h, ok - 1, ok, ew, a, b), 0.1, 0, 1, None, qa._p(sp), qa._p(sc), None, None)
+    assert rc == -1  # chi == 0
+    rc = L.tessera_qa_anneal_ex(None, 3, None, None, None, 0, None, None, 0, 0.1, 8, 1, None, qa._p(sp), qa._p(sc),
+                                None, None)
+    assert rc == -1
+
+
+def test_plain_entry_never_throws_and_reports_nan():
+    h = np.array([0.0, math.inf])
+    sp = np.zeros(2, np.int8)
+    sp[:] = 7
+    a = np.ones(3)
+    e = qa.lib().tessera_qa_anneal(qa._p(h), 2, None, None, None, 0, qa._p(a), qa._p(a), 3, 4, qa._p(sp), None)
+    assert math.isnan(e) and list(sp) == [1, 1]
+
+
+def test_solve_qa_upstream_guards():
+    L = qa.lib()
+    sp = np.zeros(30, np.int8)
+    assert L.tessera_solve_qa(None, 2, None, None, None, 0, 10, 0.1, qa._p(sp), None, None, None) == 0.0
+    e, *_ = qa.solve_qa(np.zeros(30), None, None, None, 100, 0.1)
+    assert math.isnan(e)
+
+
+def test_sv_entry_rejects_too_large_and_bad_steps():
+    L = qa.lib()
+    h = np.zeros(40)
+    sp = np.zeros(40, np.int8)
+    a = np.ones(2)
+    assert L.tessera_sv_anneal(qa._p(h), 40, None, None, None, 0, qa._p(a), qa._p(a), 2, 0.1, qa._p(sp), None, None,
+                               None, None, None) == -2
+    assert L.tessera_sv_anneal(qa._p(h), 3, None, None, None, 0, qa._p(a), qa._p(a), 0, 0.1, qa._p(sp), None, None,
+                               None, None, None) == -1
+
+
+def test_quantum_anneal_rejects_degenerate_schedule():
+    # rust/src/ffi.rs:102-104
+    with pytest.raises(qa.QaFailure) as ei:
+        qa.quantum_anneal(I.make_ising([1.0], []), I.Schedule(np.array([1.0]), np.array([0.0])), 32)
+    assert ei.value.kind is qa.QaError.BadSchedule

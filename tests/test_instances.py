@@ -1,0 +1,51 @@
+"""Seeded workload generators and the gate count of the routed swap network."""
+import numpy as np
+
+from tessera_amd import instances as I
+
+
+def routed_updates(model):
+    return sum(2 * abs(int(j) - int(i)) - 1 for i, j in zip(model.ei, model.ej) if i != j)
+
+
+def test_rng_is_deterministic_and_uniform():
+    a = [I.Rng(9).uniform() for _ in range(3)]
+    b = [I.Rng(9).uniform() for _ in range(3)]
+    assert a == b and all(0 <= x < 1 for x in a)
+    xs = np.array([I.Rng(5).normal() for _ in range(1)])
+    r = I.Rng(11)
+    zs = np.array([r.normal() for _ in range(4000)])
+    assert abs(zs.mean()) < 0.06 and abs(zs.std() - 1) < 0.06
+    assert xs.shape == (1,)
+
+
+def test_config_shapes():
+    c1 = I.config1()
+    assert c1.model.n == 12 and c1.model.m == 21 and c1.sched.steps == 200 and routed_updates(c1.model) == 41
+    c2 = I.config2(steps=3)
+    deg = np.bincount(np.concatenate([c2.model.ei, c2.model.ej]).astype(int), minlength=30)
+    assert c2.model.n == 30 and c2.model.m == 45 and np.all(deg == 3)
+    c3 = I.config3(steps=3)
+    assert c3.model.n == 128 and c3.model.m == 502 and routed_updates(c3.model) == 1998
+    assert c3.model.max_range() == 4
+    assert np.all(np.abs(c3.model.ew * (c3.model.ej - c3.model.ei).astype(float)) <= 1.0)
+    c4 = I.config4(count=20)
+    assert all(16 <= x.model.n <= 96 and x.chi in (32, 64, 128) and x.model.max_range() <= 3 for x in c4)
+
+
+def test_edges_sorted_lexicographically():
+    for m in (I.chain_range(20, 4, 1), I.random_regular3(16, 3), I.sparse_banded(30, 3, 3.0, 5)):
+        pairs = list(zip(m.ei.tolist(), m.ej.tolist()))
+        assert pairs == sorted(pairs) and all(i < j for i, j in pairs)
+
+
+def test_energy_summation_order():
+    # ising.rs:23-32: edges in list order then fields; bit-identical to a direct loop
+    m = I.chain_nnn(6, 4)
+    s = [1, -1, -1, 1, 1, -1]
+    e = 0.0
+    for i, j, w in zip(m.ei, m.ej, m.ew):
+        e -= w * s[int(i)] * s[int(j)]
+    for h, si in zip(m.h, s):
+        e -= h * si
+    assert m.energy(s) == e

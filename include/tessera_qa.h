@@ -141,6 +141,18 @@ int tessera_tebd_svd_stats(tessera_tebd* t, int64_t* out_calls, int64_t* out_swe
 /* Record HIP events around every SVD launch from now on (on != 0); resets the
  * accumulated time. */
 int tessera_tebd_set_timing(tessera_tebd* t, int on);
+/* Lockstep executor for tessera_tebd_run: 0 = off (each update is its own
+ * sequence of launches, the default), g > 0 = one cooperative launch per run
+ * call (per chunk of recorded steps) in which up to g groups of workgroups run
+ * the steps' two-site updates in super-steps of mutually independent updates,
+ * so that consecutive Trotter steps overlap; -1 = as many groups as fit on the
+ * device. Single-site half layers are folded into the two-site gates; the
+ * per-step entropy is taken from site snapshots. Results equal the launch-per-
+ * update path up to rounding. */
+int tessera_tebd_set_executor(tessera_tebd* t, int groups);
+/* Lockstep executor totals since creation: super-steps run and two-site
+ * updates placed in them (updates / (super-steps * groups) = slot fill). */
+int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* out_updates);
 /* Number of two-site updates one Trotter step performs for this model. */
 size_t tessera_tebd_updates_per_step(const tessera_tebd* t);
 void tessera_tebd_destroy(tessera_tebd* t);

@@ -134,59 +134,64 @@ int oracle_svd(const double* am, std::size_t r, std::size_t c, double* out_u, do
 }
 
 // CPU-baseline probe for one saturated two-site update at bond dimension chi:
-// theta is the 2chi x 2chi matrix of the upstream update (mps.cpp:41-100) for
-// random site tensors with bond weights exp(-k/40); the Gram eigensolver is run
-// for at most `max_sweeps` Jacobi sweeps (the full update uses up to 100).
-// Returns seconds; writes the number of sweeps actually run and the seconds
-// spent in the first sweep (for a bounded sample of the full cost).
+// theta is a full-rank 2chi x 2chi complex matrix Q1 diag(exp(-k/40)) Q2^dagger
+// (Q1, Q2 from Gram-Schmidt of seeded Gaussian matrices), factorised as the
+// upstream update does (linalg.cpp:107-172): Gram matrix, 4chi x 4chi real
+// embedding, cyclic Jacobi. The Jacobi runs at most `max_sweeps` sweeps.
+// Returns the seconds of Gram + embedding; writes the sweeps run and the seconds
+// of the Jacobi sweeps.
 double oracle_update_probe(int chi, int max_sweeps, int* out_sweeps, double* out_sweep_seconds) {
     std::mt19937_64 rng(7);
     std::normal_distribution<double> g(0.0, 1.0);
-    const int d = chi;
-    Tensor a(d, d), c(d, d);
-    for (int x = 0; x < d; ++x)
-        for (int s = 0; s < 2; ++s)
-            for (int y = 0; y < d; ++y) {
-                const double w = std::exp(-y / 40.0);
-                a.at(x, s, y) = cplx(g(rng), g(rng)) * w;
-                c.at(y, s, x) = cplx(g(rng), g(rng));
+    const std::size_t n = (std::size_t)chi * 2;
+    auto unitary = [&]() {
+        Mat q(n, n);
+        for (auto& x : q.v) x = cplx(g(rng), g(rng));
+        for (std::size_t j = 0; j < n; ++j) {
+            for (std::size_t p = 0; p < j; ++p) {
+                cplx d(0, 0);
+                for (std::size_t i = 0; i < n; ++i) d += std::conj(q.at(i, p)) * q.at(i, j);
+                for (std::size_t i = 0; i < n; ++i) q.at(i, j) -= d * q.at(i, p);
             }
-    auto t0 = std::chrono::steady_clock::now();
-    Mat th((std::size_t)d * 2, (std::size_t)d * 2);
-    for (int x = 0; x < d; ++x)
-        for (int y = 0; y < d; ++y)
-            for (int s1 = 0; s1 < 2; ++s1)
-                for (int s2 = 0; s2 < 2; ++s2) {
-                    cplx acc(0, 0);
-                    for (int k = 0; k < d; ++k) acc += a.at(x, s1, k) * c.at(k, s2, y);
-                    th.at((std::size_t)(x * 2 + s1), (std::size_t)(s2 * d + y)) = acc;
-                }
-    const std::size_t cc = th.c, rr = th.r;
-    Mat gm(cc, cc);
-    for (std::size_t i = 0; i < cc; ++i)
-        for (std::size_t j = 0; j < cc; ++j) {
+            double nr = 0;
+            for (std::size_t i = 0; i < n; ++i) nr += std::norm(q.at(i, j));
+            nr = std::sqrt(nr);
+            for (std::size_t i = 0; i < n; ++i) q.at(i, j) /= nr;
+        }
+        return q;
+    };
+    const Mat q1 = unitary(), q2 = unitary();
+    Mat th(n, n);
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t j = 0; j < n; ++j) {
             cplx acc(0, 0);
-            for (std::size_t t = 0; t < rr; ++t) acc += std::conj(th.at(t, i)) * th.at(t, j);
+            for (std::size_t k = 0; k < n; ++k) acc += q1.at(i, k) * std::exp(-(double)k / 40.0) * std::conj(q2.at(j, k));
+            th.at(i, j) = acc;
+        }
+    auto t0 = std::chrono::steady_clock::now();
+    Mat gm(n, n);
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t j = 0; j < n; ++j) {
+            cplx acc(0, 0);
+            for (std::size_t t = 0; t < n; ++t) acc += std::conj(th.at(t, i)) * th.at(t, j);
             gm.at(i, j) = acc;
         }
-    const std::size_t mm = 2 * cc;
+    const std::size_t mm = 2 * n;
     std::vector<double> emb(mm * mm, 0.0);
-    for (std::size_t i = 0; i < cc; ++i)
-        for (std::size_t j = 0; j < cc; ++j) {
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t j = 0; j < n; ++j) {
             emb[i * mm + j] = gm.at(i, j).real();
-            emb[(i + cc) * mm + (j + cc)] = gm.at(i, j).real();
-            emb[i * mm + (j + cc)] = -gm.at(i, j).imag();
-            emb[(i + cc) * mm + j] = gm.at(i, j).imag();
+            emb[(i + n) * mm + (j + n)] = gm.at(i, j).real();
+            emb[i * mm + (j + n)] = -gm.at(i, j).imag();
+            emb[(i + n) * mm + j] = gm.at(i, j).imag();
         }
     auto t1 = std::chrono::steady_clock::now();
     std::vector<double> w, v;
-    int sw = sym_jacobi(emb, mm, w, v, 1);
+    const int sw = sym_jacobi(emb, mm, w, v, max_sweeps);
     auto t2 = std::chrono::steady_clock::now();
-    if (max_sweeps > 1) sw = sym_jacobi(emb, mm, w, v, max_sweeps);
-    auto t3 = std::chrono::steady_clock::now();
     if (out_sweeps) *out_sweeps = sw;
     if (out_sweep_seconds) *out_sweep_seconds = std::chrono::duration<double>(t2 - t1).count();
-    return std::chrono::duration<double>(t1 - t0).count() + std::chrono::duration<double>(t3 - t2).count();
+    return std::chrono::duration<double>(t1 - t0).count();
 }
 
 }  // extern "C"

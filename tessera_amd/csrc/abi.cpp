@@ -243,11 +243,8 @@ int tessera_tebd_randomize(tessera_tebd* t, uint64_t seed, double decay) {
 int tessera_tebd_run(tessera_tebd* t, const double* a, const double* b, size_t count, int record) {
     try {
         if (!t || (count && (!a || !b))) throw Error(TESSERA_ERR_ARG, "null argument");
-        if (record) t->eng->ensure_trace(t->recorded + (int)count);
-        for (size_t k = 0; k < count; ++k) {
-            t->eng->step(a[k], b[k], record ? t->recorded : -1);
-            if (record) ++t->recorded;
-        }
+        t->eng->run(a, b, (int)count, record ? t->recorded : -1);
+        if (record) t->recorded += (int)count;
         return TESSERA_OK;
     } catch (const std::exception& e) {
         set_last_error(e.what());
@@ -330,6 +327,31 @@ int tessera_tebd_set_timing(tessera_tebd* t, int on) {
     try {
         if (!t) throw Error(TESSERA_ERR_ARG, "null session");
         t->eng->set_timing(on != 0);
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_set_executor(tessera_tebd* t, int groups) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        if (groups < -1) throw Error(TESSERA_ERR_ARG, "groups must be >= -1");
+        t->eng->set_executor(groups);
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* out_updates) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        const SvdStats st = t->eng->svd_stats();
+        if (out_supersteps) *out_supersteps = st.supersteps;
+        if (out_updates) *out_updates = st.exec_updates;
         return TESSERA_OK;
     } catch (const std::exception& e) {
         set_last_error(e.what());

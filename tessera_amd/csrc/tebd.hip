@@ -125,6 +125,61 @@ __device__ inline void rr_pair(int r, int k, int cp, int& i, int& j) {
     }
 }
 
+// Rows held per lane for m <= 512.
+constexpr int kRowsPerLane = 8;
+
+// Pair statistics over the rows a lane holds, summed over the wave.
+__device__ inline void pair_dots(const double2 (&x)[kRowsPerLane], const double2 (&y)[kRowsPerLane], double& al,
+                                 double& be, double& gr, double& gi) {
+    al = be = gr = gi = 0.0;
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) {
+        const double2 u = x[q], v = y[q];
+        al += u.x * u.x + u.y * u.y;
+        be += v.x * v.x + v.y * v.y;
+        gr += u.x * v.x + u.y * v.y;  // conj(u) v
+        gi += u.x * v.y - u.y * v.x;
+    }
+    al = wave_sum(al);
+    be = wave_sum(be);
+    gr = wave_sum(gr);
+    gi = wave_sum(gi);
+}
+
+// One Jacobi rotation of columns x, y (lane-held rows); returns true if applied.
+__device__ inline bool pair_rotate(double2 (&x)[kRowsPerLane], double2 (&y)[kRowsPerLane], int m) {
+    double al, be, gr, gi;
+    pair_dots(x, y, al, be, gr, gi);
+    const double gabs = sqrt(gr * gr + gi * gi);
+    if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * sqrt(al * be)) return false;
+    const double2 ph = make_double2(gr / gabs, -gi / gabs);  // conj(gamma)/|gamma|
+    const double zeta = (be - al) / (2.0 * gabs);
+    const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) {
+        const double2 u = x[q], v = cmul(y[q], ph);
+        x[q] = make_double2(cs * u.x - sn * v.x, cs * u.y - sn * v.y);
+        y[q] = make_double2(sn * u.x + cs * v.x, sn * u.y + cs * v.y);
+    }
+    return true;
+}
+
+__device__ inline void load_col(double2 (&x)[kRowsPerLane], const double2* src, int m, int lane) {
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) {
+        const int row = lane + 64 * q;
+        x[q] = row < m ? src[row] : make_double2(0, 0);
+    }
+}
+__device__ inline void store_col(const double2 (&x)[kRowsPerLane], double2* dst, int m, int lane) {
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) {
+        const int row = lane + 64 * q;
+        if (row < m) dst[row] = x[q];
+    }
+}
+
 __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, int ld, int* dims, int p, int chi,
                                                      double* __restrict__ sv_out, int* __restrict__ perm_out,
                                                      double* trunc, int use_lds, unsigned long long* stats) {
@@ -148,6 +203,11 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
         }
         __syncthreads();
     }
+    __shared__ unsigned long long s_vis, s_rot;
+    if (tid == 0) {
+        s_vis = 0;
+        s_rot = 0;
+    }
     const int cp = c + (c & 1);
     int sweeps_done = 0;
     unsigned long long visited = 0, rotated = 0;  // pair visits / rotations (wave-local, lane 0 counts)
@@ -162,31 +222,15 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
                 if (i >= c || j >= c) continue;
                 double2* x = W + (std::size_t)i * lw;
                 double2* y = W + (std::size_t)j * lw;
-                double al = 0, be = 0, gr = 0, gi = 0;
-                for (int row = lane; row < m; row += 64) {
-                    const double2 u = x[row], v = y[row];
-                    al += u.x * u.x + u.y * u.y;
-                    be += v.x * v.x + v.y * v.y;
-                    gr += u.x * v.x + u.y * v.y;  // conj(u) v
-                    gi += u.x * v.y - u.y * v.x;
-                }
-                al = wave_sum(al);
-                be = wave_sum(be);
-                gr = wave_sum(gr);
-                gi = wave_sum(gi);
-                const double gabs = sqrt(gr * gr + gi * gi);
+                double2 xa[kRowsPerLane], ya[kRowsPerLane];
+                load_col(xa, x, m, lane);
+                load_col(ya, y, m, lane);
                 ++visited;
-                if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * sqrt(al * be)) continue;
-                ++rotated;
-                if (lane == 0) s_flag = 1;
-                const double2 ph = make_double2(gr / gabs, -gi / gabs);  // conj(gamma)/|gamma|
-                const double zeta = (be - al) / (2.0 * gabs);
-                const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
-                for (int row = lane; row < m; row += 64) {
-                    const double2 u = x[row], v = cmul(y[row], ph);
-                    x[row] = make_double2(cs * u.x - sn * v.x, cs * u.y - sn * v.y);
-                    y[row] = make_double2(sn * u.x + cs * v.x, sn * u.y + cs * v.y);
+                if (pair_rotate(xa, ya, m)) {
+                    ++rotated;
+                    if (lane == 0) s_flag = 1;
+                    store_col(xa, x, m, lane);
+                    store_col(ya, y, m, lane);
                 }
             }
             __syncthreads();
@@ -227,17 +271,20 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
             const double s = s_nrm[s_perm[q]];
             disc += s * s;
         }
-        if (total > 0) {
-            const double tr = disc / total;
-            if (*trunc < tr) *trunc = tr;
-        }
+        if (total > 0) trunc[blockIdx.x] = fmax(trunc[blockIdx.x], disc / total);
         dims[p + 1] = keep;
-        stats[0] += 1;
-        stats[1] += sweeps_done;
     }
     if (lane == 0) {
-        atomicAdd(&stats[2], visited * (unsigned long long)m);
-        atomicAdd(&stats[3], rotated * (unsigned long long)m);
+        atomicAdd(&s_vis, visited * (unsigned long long)m);  // LDS
+        atomicAdd(&s_rot, rotated * (unsigned long long)m);
+    }
+    __syncthreads();
+    if (tid == 0) {  // this workgroup's accounting slot
+        unsigned long long* slot = stats + 4 * (std::size_t)blockIdx.x;
+        slot[0] += 1;
+        slot[1] += (unsigned long long)sweeps_done;
+        slot[2] += s_vis;
+        slot[3] += s_rot;
     }
     for (int q = tid; q < c; q += blockDim.x) {
         sv_out[q] = s_nrm[s_perm[q]];
@@ -259,71 +306,69 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
 // formulas, FP64 angles and the convergence test are those of k_svd.
 constexpr int kBlk = 8;
 constexpr int kDistThreads = 512;  // 8 waves: one column pair per wave
+constexpr int kStatSlots = 1024;   // per-workgroup accounting slots (largest grid)
 
-__device__ inline void jacobi_pair_lds(double2* x, double2* y, int m, int lane, bool& rotated,
-                                       unsigned long long& nvis, unsigned long long& nrot) {
-    double al = 0, be = 0, gr = 0, gi = 0;
-    for (int row = lane; row < m; row += 64) {
-        const double2 u = x[row], v = y[row];
-        al += u.x * u.x + u.y * u.y;
-        be += v.x * v.x + v.y * v.y;
-        gr += u.x * v.x + u.y * v.y;
-        gi += u.x * v.y - u.y * v.x;
-    }
-    al = wave_sum(al);
-    be = wave_sum(be);
-    gr = wave_sum(gr);
-    gi = wave_sum(gi);
-    const double gabs = sqrt(gr * gr + gi * gi);
-    ++nvis;
-    if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * sqrt(al * be)) return;
-    ++nrot;
-    rotated = true;
-    const double2 ph = make_double2(gr / gabs, -gi / gabs);
-    const double zeta = (be - al) / (2.0 * gabs);
-    const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
-    for (int row = lane; row < m; row += 64) {
-        const double2 u = x[row], v = cmul(y[row], ph);
-        x[row] = make_double2(cs * u.x - sn * v.x, cs * u.y - sn * v.y);
-        y[row] = make_double2(sn * u.x + cs * v.x, sn * u.y + cs * v.y);
-    }
-}
+// Shared LDS of the block-Jacobi SVD (static part).
+struct SvdShared {
+    double nrm[kMaxCols];
+    int perm[kMaxCols];
+    int rot;
+    unsigned long long vis, rots;
+};
 
-__global__ void __launch_bounds__(kDistThreads) k_svd_dist(double2* __restrict__ Wg, int ld, int* dims, int p,
-                                                           int chi, double* __restrict__ sv_out,
-                                                           int* __restrict__ perm_out, double* trunc, int* flags,
-                                                           unsigned long long* stats) {
-    namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
-    extern __shared__ double2 lds[];
-    __shared__ double s_nrm[kMaxCols];
-    __shared__ int s_perm[kMaxCols];
-    __shared__ int s_rot;
+// How a block-Jacobi SVD is synchronised with others in the same launch. Alone:
+// every sync() is its own, rounds and sweeps follow its own matrix. Lockstep:
+// several SVDs (one per group of workgroups) share one grid barrier, so every
+// group runs `rounds` outer steps per sweep and the sweep loop ends when no
+// group rotated (`gflags`, cleared by the grid leader); a group whose own sweep
+// rotated nothing stops working but keeps joining the barriers.
+struct SvdMode {
+    int rounds = -1;            // < 0: own tournament length
+    int* gflags = nullptr;      // grid-wide rotation flags, [2]
+    bool grid_leader = false;   // clears gflags
+    bool active = true;         // false: this group has no matrix
+};
+
+// Block-Jacobi SVD run by `nworkers` workgroups that synchronise with `sync()`.
+// Worker `me` < P takes block pair `me` of each tournament round; workers with
+// me >= P only join the barriers. The leader (me == 0) sorts, truncates and
+// writes dims[p+1], sv_out, perm_out; callers must synchronise again before
+// reading them. Accounting goes to this workgroup's slots: stats[4*b + 0..3] =
+// calls, sweeps, visited and rotated pair-rows, trunc[b] = max discarded weight
+// (b = blockIdx.x; the host reduces the slots, no cross-workgroup atomics).
+template <class Sync>
+__device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, int chi, double* __restrict__ sv_out,
+                           int* __restrict__ perm_out, double* trunc, int* flags, unsigned long long* stats, int me,
+                           int nworkers, Sync& sync, double2* lds, SvdShared& sh, const SvdMode& mode) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int Dl = dims[p], Dr = dims[p + 2];
+    const int Dl = mode.active ? dims[p] : 1, Dr = mode.active ? dims[p + 2] : 1;
     const int R = 2 * Dl, C = 2 * Dr;
     const bool tall = C <= R;
     const int m = tall ? R : C, c = tall ? C : R;
     const int nb = (c + kBlk - 1) / kBlk;
     const int nbp = nb + (nb & 1) < 2 ? 2 : nb + (nb & 1);
     const int P = nbp / 2;
-    const int me = blockIdx.x;
+    const int rounds = mode.rounds >= 0 ? mode.rounds : nbp - 1;
     if (me == 0 && tid == 0) {
         flags[0] = 0;
         flags[1] = 0;
     }
-    grid.sync();
+    if (tid == 0) {
+        sh.vis = 0;
+        sh.rots = 0;
+    }
+    sync();
+    bool working = mode.active;
     int sweeps_done = 0;
     unsigned long long nvis = 0, nrot = 0;
     for (int sweep = 0; sweep < 40; ++sweep) {
-        ++sweeps_done;
-        for (int r = 0; r < nbp - 1; ++r) {
-            if (me < P) {
+        if (working) ++sweeps_done;
+        for (int r = 0; r < rounds; ++r) {
+            if (working && r < nbp - 1 && me < P) {
                 int bi, bj;
                 rr_pair(r, me, nbp, bi, bj);
-                if (tid == 0) s_rot = 0;
-                // load the two blocks: LDS column q <-> block (q < 8 ? bi : bj), member q & 7
+                if (tid == 0) sh.rot = 0;
+                // LDS column q <-> block (q < 8 ? bi : bj), member q & 7
                 for (int q = wave; q < 2 * kBlk; q += kDistThreads / 64) {
                     const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
                     double2* dst = lds + (std::size_t)q * m;
@@ -334,28 +379,55 @@ __global__ void __launch_bounds__(kDistThreads) k_svd_dist(double2* __restrict__
                 }
                 __syncthreads();
                 bool rotated = false;
+                double2 xa[kRowsPerLane], ya[kRowsPerLane];
                 if (r == 0) {  // pairs inside each block, once per sweep
                     for (int s = 0; s < kBlk - 1; ++s) {
                         const int half = wave >> 2, k = wave & 3;  // waves 0-3 block bi, 4-7 block bj
                         int i, j;
                         rr_pair(s, k, kBlk, i, j);
                         const int blk = half ? bj : bi;
-                        if (blk * kBlk + i < c && blk * kBlk + j < c)
-                            jacobi_pair_lds(lds + (std::size_t)(half * kBlk + i) * m,
-                                            lds + (std::size_t)(half * kBlk + j) * m, m, lane, rotated, nvis, nrot);
+                        if (blk * kBlk + i < c && blk * kBlk + j < c) {
+                            double2* px = lds + (std::size_t)(half * kBlk + i) * m;
+                            double2* py = lds + (std::size_t)(half * kBlk + j) * m;
+                            load_col(xa, px, m, lane);
+                            load_col(ya, py, m, lane);
+                            if (pair_rotate(xa, ya, m)) {
+                                rotated = true;
+                                ++nrot;
+                                store_col(xa, px, m, lane);
+                                store_col(ya, py, m, lane);
+                            }
+                            ++nvis;
+                        }
                         __syncthreads();
                     }
                 }
-                for (int s = 0; s < kBlk; ++s) {  // cross pairs (bi member w, bj member w+s)
-                    const int i = wave, j = (wave + s) & (kBlk - 1);
-                    if (bi * kBlk + i < c && bj * kBlk + j < c)
-                        jacobi_pair_lds(lds + (std::size_t)i * m, lds + (std::size_t)(kBlk + j) * m, m, lane,
-                                        rotated, nvis, nrot);
+                // cross pairs: wave w keeps member w of block bi in registers and
+                // meets member (w + s) & 7 of block bj in round s
+                const bool mine = bi * kBlk + wave < c;
+                if (mine) load_col(xa, lds + (std::size_t)wave * m, m, lane);
+                for (int s = 0; s < kBlk; ++s) {
+                    const int j = (wave + s) & (kBlk - 1);
+                    if (mine && bj * kBlk + j < c) {
+                        double2* py = lds + (std::size_t)(kBlk + j) * m;
+                        load_col(ya, py, m, lane);
+                        if (pair_rotate(xa, ya, m)) {
+                            rotated = true;
+                            ++nrot;
+                            store_col(ya, py, m, lane);
+                        }
+                        ++nvis;
+                    }
                     __syncthreads();
                 }
-                if (rotated && lane == 0) s_rot = 1;
+                if (mine) store_col(xa, lds + (std::size_t)wave * m, m, lane);
                 __syncthreads();
-                if (tid == 0 && s_rot) atomicOr(&flags[sweep & 1], 1);
+                if (rotated && lane == 0) sh.rot = 1;
+                __syncthreads();
+                if (tid == 0 && sh.rot) {
+                    atomicOr(&flags[sweep & 1], 1);
+                    if (mode.gflags) atomicOr(&mode.gflags[sweep & 1], 1);
+                }
                 for (int q = wave; q < 2 * kBlk; q += kDistThreads / 64) {
                     const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
                     const double2* src = lds + (std::size_t)q * m;
@@ -365,73 +437,101 @@ __global__ void __launch_bounds__(kDistThreads) k_svd_dist(double2* __restrict__
                     }
                 }
             }
-            grid.sync();
+            sync();
         }
         const int f = __hip_atomic_load(&flags[sweep & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        grid.sync();
+        const int gf = mode.gflags ? __hip_atomic_load(&mode.gflags[sweep & 1], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT)
+                                   : f;
+        sync();
         if (me == 0 && tid == 0) flags[sweep & 1] = 0;
-        if (!f) break;
+        if (mode.grid_leader && tid == 0 && mode.gflags) mode.gflags[sweep & 1] = 0;
+        if (!f) working = false;
+        if (!gf) break;
     }
     if (lane == 0 && nvis) {
-        atomicAdd(&stats[2], nvis * (unsigned long long)m);
-        atomicAdd(&stats[3], nrot * (unsigned long long)m);
+        atomicAdd(&sh.vis, nvis * (unsigned long long)m);  // LDS
+        atomicAdd(&sh.rots, nrot * (unsigned long long)m);
     }
-    // column norms -> sv_out (scratch), then workgroup 0 sorts and truncates
-    for (int col = me * (kDistThreads / 64) + wave; col < c; col += gridDim.x * (kDistThreads / 64)) {
-        const double2* x = Wg + (std::size_t)col * ld;
-        double s = 0.0;
-        for (int row = lane; row < m; row += 64) s += x[row].x * x[row].x + x[row].y * x[row].y;
-        s = wave_sum(s);
-        if (lane == 0) sv_out[col] = sqrt(s);
+    // column norms -> sv_out (scratch), then the leader sorts and truncates
+    if (mode.active)
+        for (int col = me * (kDistThreads / 64) + wave; col < c; col += nworkers * (kDistThreads / 64)) {
+            const double2* x = Wg + (std::size_t)col * ld;
+            double s = 0.0;
+            for (int row = lane; row < m; row += 64) s += x[row].x * x[row].x + x[row].y * x[row].y;
+            s = wave_sum(s);
+            if (lane == 0) sv_out[col] = sqrt(s);
+        }
+    sync();
+    unsigned long long* slot = stats + 4 * (std::size_t)blockIdx.x;
+    if (tid == 0) {
+        slot[2] += sh.vis;
+        slot[3] += sh.rots;
     }
-    grid.sync();
-    if (me != 0) return;
-    for (int q = tid; q < c; q += blockDim.x) s_nrm[q] = sv_out[q];
+    if (me != 0 || !mode.active) return;
+    for (int q = tid; q < c; q += blockDim.x) sh.nrm[q] = sv_out[q];
     __syncthreads();
     for (int q = tid; q < c; q += blockDim.x) {
-        const double v = s_nrm[q];
+        const double v = sh.nrm[q];
         int rank = 0;
         for (int o = 0; o < c; ++o) {
-            const double u = s_nrm[o];
+            const double u = sh.nrm[o];
             rank += (u > v) || (u == v && o < q);
         }
-        s_perm[rank] = q;
+        sh.perm[rank] = q;
     }
     __syncthreads();
     if (tid == 0) {
         double total = 0.0;
         for (int q = 0; q < c; ++q) {
-            const double s = s_nrm[s_perm[q]];
+            const double s = sh.nrm[sh.perm[q]];
             total += s * s;
         }
         int keep = min(chi, c);
-        while (keep > 1 && s_nrm[s_perm[keep - 1]] < 1e-14) --keep;
+        while (keep > 1 && sh.nrm[sh.perm[keep - 1]] < 1e-14) --keep;
         double disc = 0.0;
         for (int q = keep; q < c; ++q) {
-            const double s = s_nrm[s_perm[q]];
+            const double s = sh.nrm[sh.perm[q]];
             disc += s * s;
         }
-        if (total > 0) {
-            const double tr = disc / total;
-            if (*trunc < tr) *trunc = tr;
-        }
+        if (total > 0) trunc[blockIdx.x] = fmax(trunc[blockIdx.x], disc / total);
         dims[p + 1] = keep;
-        stats[0] += 1;
-        stats[1] += sweeps_done;
+        slot[0] += 1;
+        slot[1] += (unsigned long long)sweeps_done;
     }
     for (int q = tid; q < c; q += blockDim.x) {
-        sv_out[q] = s_nrm[s_perm[q]];
-        perm_out[q] = s_perm[q];
+        sv_out[q] = sh.nrm[sh.perm[q]];
+        perm_out[q] = sh.perm[q];
     }
+}
+
+struct GridSync {
+    __device__ void operator()() const { cooperative_groups::this_grid().sync(); }
+};
+
+__global__ void __launch_bounds__(kDistThreads) k_svd_dist(double2* __restrict__ Wg, int ld, int* dims, int p,
+                                                           int chi, double* __restrict__ sv_out,
+                                                           int* __restrict__ perm_out, double* trunc, int* flags,
+                                                           unsigned long long* stats, int xcd_stride) {
+    extern __shared__ double2 lds[];
+    __shared__ SvdShared sh;
+    GridSync sync;
+    // With xcd_stride = 8 only every 8th workgroup works; workgroups are dealt to
+    // the XCDs round-robin, so the working ones share one XCD and its L2.
+    const bool worker = (blockIdx.x % xcd_stride) == 0;
+    const int nworkers = (int)(gridDim.x / xcd_stride);
+    const int me = worker ? (int)(blockIdx.x / xcd_stride) : nworkers + (int)blockIdx.x;
+    const SvdMode mode;
+    svd_blocks(Wg, ld, dims, p, chi, sv_out, perm_out, trunc, flags, stats, me, nworkers, sync, lds, sh, mode);
 }
 
 // ---- reconstruction (mps.cpp:87-99) ---------------------------------------
 // tall (M = theta):        A'[row,k] = W[:,perm k][row]/S_k;  C'[k,col] = sum_row conj(U[row,k]) M[row,col]
 // wide (M = theta^dagger): C'[k,col] = conj(W[col, perm k]);  A'[row,k] = sum_col conj(M[col,row]) W[col,perm k] / S_k^2
-__global__ void __launch_bounds__(256) k_recon(double2* sites, const std::size_t* off, const int* cap,
-                                               const int* dims, int p, const double2* __restrict__ M,
-                                               const double2* __restrict__ W, int ld,
-                                               const double* __restrict__ sv, const int* __restrict__ perm) {
+__device__ void recon_part(double2* sites, const std::size_t* off, const int* cap, const int* dims, int p,
+                           const double2* __restrict__ M, const double2* __restrict__ W, int ld,
+                           const double* __restrict__ sv, const int* __restrict__ perm, long long gid,
+                           long long stride) {
     const int Dl = dims[p], keep = dims[p + 1], Dr = dims[p + 2];
     const int R = 2 * Dl, C = 2 * Dr;
     const bool tall = C <= R;
@@ -440,8 +540,6 @@ __global__ void __launch_bounds__(256) k_recon(double2* sites, const std::size_t
     double2* Cn = sites + off[p + 1];
     const long long ndirect = tall ? (long long)R * keep : (long long)keep * C;
     const long long ngemm = tall ? (long long)keep * C : (long long)R * keep;
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long idx = gid; idx < ndirect + ngemm; idx += stride) {
         if (idx < ndirect) {
             if (tall) {
@@ -478,8 +576,67 @@ __global__ void __launch_bounds__(256) k_recon(double2* sites, const std::size_t
     }
 }
 
+__global__ void __launch_bounds__(256) k_recon(double2* sites, const std::size_t* off, const int* cap,
+                                               const int* dims, int p, const double2* __restrict__ M,
+                                               const double2* __restrict__ W, int ld,
+                                               const double* __restrict__ sv, const int* __restrict__ perm) {
+    recon_part(sites, off, cap, dims, p, M, W, ld, sv, perm, (long long)blockIdx.x * blockDim.x + threadIdx.x,
+               (long long)gridDim.x * blockDim.x);
+}
+
 // ---- environments (mps.cpp:150-178, 218-262) ---------------------------------
 // Right: T[a',s,b] = sum_b' R[b,b'] conj(A(a',s,b'));  Rout[a,a'] = sum_{s,b} A(a,s,b) T[a',s,b]
+__device__ void envr1_part(const double2* sites, std::size_t o, int st, int Dl, int Dr, const double2* Rin, int ldr,
+                           double2* T, long long gid, long long stride) {
+    const long long tot = (long long)Dl * 2 * Dr;
+    for (long long idx = gid; idx < tot; idx += stride) {
+        const int ap = (int)(idx / (2 * Dr)), s = (int)((idx / Dr) % 2), b = (int)(idx % Dr);
+        double2 acc = make_double2(0, 0);
+        for (int bp = 0; bp < Dr; ++bp)
+            acc = cadd(acc, cmul(Rin[(std::size_t)b * ldr + bp], cconj(ld_site(sites, o, st, ap, s, bp))));
+        T[idx] = acc;
+    }
+}
+__device__ void envr2_part(const double2* sites, std::size_t o, int st, int Dl, int Dr, const double2* T,
+                           double2* Rout, int ldr, long long gid, long long stride) {
+    const long long tot = (long long)Dl * Dl;
+    for (long long idx = gid; idx < tot; idx += stride) {
+        const int a = (int)(idx / Dl), ap = (int)(idx % Dl);
+        double2 acc = make_double2(0, 0);
+        for (int s = 0; s < 2; ++s)
+            for (int b = 0; b < Dr; ++b)
+                acc = cadd(acc, cmul(ld_site(sites, o, st, a, s, b), T[((std::size_t)ap * 2 + s) * Dr + b]));
+        Rout[(std::size_t)a * ldr + ap] = acc;
+    }
+}
+// Closed-form qubit-0 entropy from site 0 (Dl = 1, Dr) and the right
+// environment of site 1 (mps.cpp:263-290).
+__device__ double rho0_entropy(const double2* sites, std::size_t o, int st, int Dr, const double2* R1, int ldr) {
+    double2 rho[2][2];
+    for (int s = 0; s < 2; ++s)
+        for (int sp = 0; sp < 2; ++sp) {
+            double2 acc = make_double2(0, 0);
+            for (int b = 0; b < Dr; ++b)
+                for (int bp = 0; bp < Dr; ++bp)
+                    acc = cadd(acc, cmul(cmul(ld_site(sites, o, st, 0, s, b), cconj(ld_site(sites, o, st, 0, sp, bp))),
+                                         R1[(std::size_t)b * ldr + bp]));
+            rho[s][sp] = acc;
+        }
+    const double tr = rho[0][0].x + rho[1][1].x;
+    double ent = 0.0;
+    if (tr > 0) {
+        const double a = rho[0][0].x / tr, d = rho[1][1].x / tr;
+        const double2 of = make_double2(rho[0][1].x / tr, rho[0][1].y / tr);
+        const double off2 = of.x * of.x + of.y * of.y;
+        const double mean = (a + d) / 2.0;
+        const double rad = sqrt(fmax(0.0, ((a - d) / 2) * ((a - d) / 2) + off2));
+        const double l1 = fmin(1.0, fmax(0.0, mean + rad)), l2 = fmin(1.0, fmax(0.0, mean - rad));
+        if (l1 > 1e-12) ent -= l1 * log2(l1);
+        if (l2 > 1e-12) ent -= l2 * log2(l2);
+    }
+    return ent;
+}
+
 __global__ void __launch_bounds__(256) k_envr1(const double2* sites, const std::size_t* off, const int* cap,
                                                const int* dims, int j, const double2* Rin, int ldr, double2* T) {
     const int Dl = dims[j], Dr = dims[j + 1], st = cap[j + 1];
@@ -654,6 +811,187 @@ __global__ void k_randomize(double2* sites, const std::size_t* off, const int* c
     }
 }
 
+// theta of update p by 16x16 tiles (the computation of k_theta); each workgroup
+// computes two tiles at a time (thread halves), tiles dealt round-robin over the
+// group's workgroups.
+__device__ void theta_tiles(const double2* __restrict__ sites, const std::size_t* off, const int* cap,
+                            const int* dims, int p, const Gate4& g, double2* __restrict__ M, double2* __restrict__ W,
+                            int ld, int rank, int nwg, double2* tl) {
+    const int Dl = dims[p], mid = dims[p + 1], Dr = dims[p + 2];
+    const int sA = cap[p + 1], sC = cap[p + 2];
+    const std::size_t oA = off[p], oC = off[p + 1];
+    const int half = threadIdx.x >> 8, t16 = threadIdx.x & 255, tx = t16 & 15, ty = t16 >> 4;
+    double2* As = tl + half * (2 * 16 * 2 * 17);  // [16][2][17]
+    double2* Cs = As + 16 * 2 * 17;
+    const int nta = (Dl + 15) / 16, ntb = (Dr + 15) / 16, total = nta * ntb;
+    const bool tall = Dr <= Dl;
+    for (int base = rank * 2; base < total; base += nwg * 2) {
+        const int tile = base + half;
+        const bool active = tile < total;
+        const int ta = active ? tile / ntb : 0, tb = active ? tile % ntb : 0;
+        const int a = ta * 16 + ty, b = tb * 16 + tx;
+        double2 th[2][2] = {{make_double2(0, 0), make_double2(0, 0)}, {make_double2(0, 0), make_double2(0, 0)}};
+        for (int c0 = 0; c0 < mid; c0 += 16) {
+            for (int s = 0; s < 2; ++s) {
+                const int cc = c0 + tx, cr = c0 + ty;
+                As[(ty * 2 + s) * 17 + tx] =
+                    (active && a < Dl && cc < mid) ? ld_site(sites, oA, sA, a, s, cc) : make_double2(0, 0);
+                Cs[(ty * 2 + s) * 17 + tx] =
+                    (active && cr < mid && b < Dr) ? ld_site(sites, oC, sC, cr, s, b) : make_double2(0, 0);
+            }
+            __syncthreads();
+            for (int cc = 0; cc < 16; ++cc)
+                for (int s1 = 0; s1 < 2; ++s1) {
+                    const double2 x = As[(ty * 2 + s1) * 17 + cc];
+                    th[s1][0] = cadd(th[s1][0], cmul(x, Cs[(cc * 2 + 0) * 17 + tx]));
+                    th[s1][1] = cadd(th[s1][1], cmul(x, Cs[(cc * 2 + 1) * 17 + tx]));
+                }
+            __syncthreads();
+        }
+        if (active && a < Dl && b < Dr)
+            for (int r1 = 0; r1 < 2; ++r1)
+                for (int r2 = 0; r2 < 2; ++r2) {
+                    double2 acc = make_double2(0, 0);
+                    for (int s1 = 0; s1 < 2; ++s1)
+                        for (int s2 = 0; s2 < 2; ++s2)
+                            acc = cadd(acc, cmul(g.g[(r1 * 2 + r2) * 4 + s1 * 2 + s2], th[s1][s2]));
+                    const int row = a * 2 + r1, col = r2 * Dr + b;
+                    const std::size_t at = tall ? (std::size_t)col * ld + row : (std::size_t)row * ld + col;
+                    const double2 v = tall ? acc : cconj(acc);
+                    M[at] = v;
+                    W[at] = v;
+                }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lockstep executor. One cooperative launch holds G groups of W workgroups
+// (group g = blockIdx % G; with G a multiple of 8 a group sits on one XCD, as
+// workgroups are dealt to the 8 XCDs round-robin). The host schedules the
+// two-site updates of several Trotter steps into super-steps of at most G
+// mutually independent updates (an update runs only after every earlier update
+// on either of its sites), so consecutive steps overlap. Each super-step is
+//   1. site snapshots due at this super-step (all workgroups), theta of each
+//      group's update (its workgroups);
+//   2. block-Jacobi SVDs of all groups in lockstep (one grid barrier per
+//      tournament round, sweeps until no group rotates);
+//   3. the group leaders sort and truncate;
+//   4. reconstruction of each group's two sites.
+// Only the cooperative-groups grid barrier synchronises workgroups. A snapshot
+// copies a site after the last update of its step and before the next update
+// touching it changes the site (sites change only in phase 4, bond dims at the
+// end of phase 3), with the single-site gates still pending applied.
+struct ExecUpdate {
+    int site;   // -1: idle slot
+    Gate4 g;    // two-site gate with pending single-site gates folded in
+};
+struct ExecSnap {
+    int site;
+    int slot;       // snapshot index (recorded step)
+    double2 g[4];   // 2x2 gate applied to the copy
+};
+struct ExecArgs {
+    const ExecUpdate* ups;     // [nsteps][G]
+    const int* rounds;         // [nsteps] tournament rounds of the widest SVD
+    const int* snap_begin;     // [nsteps + 1]
+    const ExecSnap* snaps;
+    int nsteps;
+    int G;
+    double2* sites;
+    const std::size_t* off;
+    const int* cap;
+    int* dims;
+    int n, chi;
+    double2* mat;
+    double2* work;
+    int ld;
+    double* sv;
+    int* perm;
+    int* svd_flags;     // [G][2]
+    int* gflags;        // [2]
+    double2* snap;
+    std::size_t snap_stride;
+    int* snap_dims;     // [slots][n + 1]
+    double* trunc;      // per-workgroup slots
+    unsigned long long* stats;
+};
+
+__global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
+    extern __shared__ double2 lds[];
+    __shared__ SvdShared sh;
+    const int tid = threadIdx.x;
+    const int G = a.G;
+    const int grp = (int)(blockIdx.x % G), rank = (int)(blockIdx.x / G), nwg = (int)(gridDim.x / G);
+    GridSync sync;
+    const std::size_t mat_sz = (std::size_t)a.ld * a.ld;
+    double2* M = a.mat + grp * mat_sz;
+    double2* W = a.work + grp * mat_sz;
+    double* sv = a.sv + (std::size_t)grp * a.ld;
+    int* perm = a.perm + (std::size_t)grp * a.ld;
+    int* sflags = a.svd_flags + 2 * grp;
+    const long long gid = (long long)rank * blockDim.x + tid, gstride = (long long)nwg * blockDim.x;
+    const long long tid_all = (long long)blockIdx.x * blockDim.x + tid;
+    const long long all = (long long)gridDim.x * blockDim.x;
+    if (blockIdx.x == 0 && tid == 0) {
+        a.gflags[0] = 0;
+        a.gflags[1] = 0;
+    }
+    for (int st = 0; st < a.nsteps; ++st) {
+        const ExecUpdate& u = a.ups[(std::size_t)st * G + grp];
+        const bool active = u.site >= 0;
+        // 1. snapshots (whole grid), theta (group)
+        for (int q = a.snap_begin[st]; q < a.snap_begin[st + 1]; ++q) {
+            const ExecSnap& sn = a.snaps[q];
+            const int i = sn.site;
+            const int Dl = a.dims[i], Dr = a.dims[i + 1], stc = a.cap[i + 1];
+            const double2* src = a.sites + a.off[i];
+            double2* dst = a.snap + (std::size_t)sn.slot * a.snap_stride + a.off[i];
+            for (long long idx = tid_all; idx < (long long)Dl * Dr; idx += all) {
+                const int x = (int)(idx / Dr), b = (int)(idx % Dr);
+                const double2 v0 = src[((std::size_t)x * 2 + 0) * stc + b];
+                const double2 v1 = src[((std::size_t)x * 2 + 1) * stc + b];
+                dst[((std::size_t)x * 2 + 0) * stc + b] = cadd(cmul(sn.g[0], v0), cmul(sn.g[1], v1));
+                dst[((std::size_t)x * 2 + 1) * stc + b] = cadd(cmul(sn.g[2], v0), cmul(sn.g[3], v1));
+            }
+            if (tid_all == 0) {
+                int* sd = a.snap_dims + (std::size_t)sn.slot * (a.n + 1);
+                sd[i] = Dl;
+                sd[i + 1] = Dr;
+            }
+        }
+        if (active) theta_tiles(a.sites, a.off, a.cap, a.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
+        // 2-3. lockstep SVD; svd_blocks opens with a grid barrier
+        SvdMode mode;
+        mode.rounds = a.rounds[st];
+        mode.gflags = a.gflags;
+        mode.grid_leader = blockIdx.x == 0;
+        mode.active = active;
+        svd_blocks(W, a.ld, a.dims, active ? u.site : 0, a.chi, sv, perm, a.trunc, sflags, a.stats, rank, nwg, sync,
+                   lds, sh, mode);
+        sync();
+        // 4. reconstruction
+        if (active) recon_part(a.sites, a.off, a.cap, a.dims, u.site, M, W, a.ld, sv, perm, gid, gstride);
+        sync();
+    }
+}
+
+// Applies one 2x2 gate per site (flushes the executor's pending single-site gates).
+__global__ void __launch_bounds__(256) k_apply_site_gates(double2* sites, const std::size_t* off, const int* cap,
+                                                          const int* dims, const double2* gates) {
+    const int i = blockIdx.x;
+    const int Dl = dims[i], Dr = dims[i + 1], st = cap[i + 1];
+    const double2 g00 = gates[4 * i], g01 = gates[4 * i + 1], g10 = gates[4 * i + 2], g11 = gates[4 * i + 3];
+    double2* t = sites + off[i];
+    for (int idx = threadIdx.x; idx < Dl * Dr; idx += blockDim.x) {
+        const int x = idx / Dr, b = idx % Dr;
+        double2* p0 = t + ((std::size_t)x * 2 + 0) * st + b;
+        double2* p1 = t + ((std::size_t)x * 2 + 1) * st + b;
+        const double2 v0 = *p0, v1 = *p1;
+        *p0 = cadd(cmul(g00, v0), cmul(g01, v1));
+        *p1 = cadd(cmul(g10, v0), cmul(g11, v1));
+    }
+}
+
 int64_t grid_for(long long work, int block) {
     long long g = (work + block - 1) / block;
     if (g < 1) g = 1;
@@ -692,13 +1030,13 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     const int ld = 2 * maxcap;
     ld_ = ld;
     flags_.alloc(2);
-    stats_.alloc(4);
-    TQ_HIP(hipMemset(stats_.p, 0, 4 * sizeof(unsigned long long)));
+    stats_.alloc(4 * (std::size_t)kStatSlots);
+    TQ_HIP(hipMemset(stats_.p, 0, 4 * (std::size_t)kStatSlots * sizeof(unsigned long long)));
     mat_.alloc((std::size_t)ld * ld);
     work_.alloc((std::size_t)ld * ld);
     sv_.alloc(ld);
     perm_.alloc(ld);
-    trunc_.alloc(1);
+    trunc_.alloc(kStatSlots);
     h_.alloc(n_);
     TQ_HIP(hipMemcpy(h_.p, model_.h.data(), n_ * sizeof(double), hipMemcpyHostToDevice));
     if (!model_.bias.empty()) {
@@ -715,8 +1053,11 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     TQ_HIP(hipMemcpy(off_dev_.p, off_.data(), (n_ + 1) * sizeof(std::size_t), hipMemcpyHostToDevice));
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd_dist, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
+    TQ_HIP(hipFuncSetAttribute((const void*)k_tebd_lockstep, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     const char* fd = std::getenv("TESSERA_FORCE_DIST_SVD");  // test knob: distributed SVD at every size
     force_dist_ = fd && fd[0] == '1';
+    const char* xp = std::getenv("TESSERA_SVD_XCD_PACK");  // 1: keep the SVD's workgroups on one XCD
+    xcd_stride_ = (xp && xp[0] == '1') ? 8 : 1;
     build_plan();
     reset_product();
 }
@@ -764,8 +1105,7 @@ void Tebd::reset_product() {
     TQ_HIP(hipMemcpyAsync(sites_.p, host.data(), host.size() * sizeof(double2), hipMemcpyHostToDevice, stream_));
     std::vector<int> ones(n_ + 1, 1);
     TQ_HIP(hipMemcpyAsync(dims_.p, ones.data(), ones.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
-    const double zero = 0.0;
-    TQ_HIP(hipMemcpyAsync(trunc_.p, &zero, sizeof(double), hipMemcpyHostToDevice, stream_));
+    TQ_HIP(hipMemsetAsync(trunc_.p, 0, kStatSlots * sizeof(double), stream_));
     TQ_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -777,8 +1117,7 @@ void Tebd::randomize(uint64_t seed, double decay) {
                        (unsigned long long)seed, decay);
     TQ_LAUNCH_CHECK();
     TQ_HIP(hipMemcpyAsync(dims_.p, cap_.data(), (n_ + 1) * sizeof(int), hipMemcpyHostToDevice, stream_));
-    const double zero = 0.0;
-    TQ_HIP(hipMemcpyAsync(trunc_.p, &zero, sizeof(double), hipMemcpyHostToDevice, stream_));
+    TQ_HIP(hipMemsetAsync(trunc_.p, 0, kStatSlots * sizeof(double), stream_));
     TQ_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -808,13 +1147,15 @@ void Tebd::two_site(int p, const Gate4& g) {
         const int ccap = std::min(2 * Dlc, 2 * Drc), mcap = std::max(2 * Dlc, 2 * Drc);
         const int nb = (ccap + kBlk - 1) / kBlk;
         const int nbp = std::max(2, nb + (nb & 1));
-        const unsigned nwg = (unsigned)(nbp / 2);
+        const unsigned nwg = (unsigned)(nbp / 2) * (unsigned)xcd_stride_;
         const std::size_t shm = (std::size_t)2 * kBlk * mcap * sizeof(double2);
         int* flags = flags_.p;
         int pp = p, cc = chi_;
         unsigned long long* stats = stats_.p;
-        void* args[] = {(void*)&work_.p, (void*)&ld_,     (void*)&dims_.p,  (void*)&pp,    (void*)&cc,
-                        (void*)&sv_.p,   (void*)&perm_.p, (void*)&trunc_.p, (void*)&flags, (void*)&stats};
+        int xs = xcd_stride_;
+        void* args[] = {(void*)&work_.p,  (void*)&ld_,     (void*)&dims_.p, (void*)&pp,
+                        (void*)&cc,       (void*)&sv_.p,   (void*)&perm_.p, (void*)&trunc_.p,
+                        (void*)&flags,    (void*)&stats,   (void*)&xs};
         TQ_HIP(hipLaunchCooperativeKernel((const void*)k_svd_dist, dim3(nwg), dim3(kDistThreads), args, shm, stream_));
     }
     if (timing_) TQ_HIP(hipEventRecord(events_[ev_next_++].second, stream_));
@@ -873,15 +1214,287 @@ void Tebd::step(double A, double B, int rec) {
     if (rec >= 0) entropy_sweep(rec);
 }
 
-void Tebd::env_right(const double2* rin, int j, double2* rout) {
+void Tebd::run(const double* a, const double* b, int count, int rec0) {
+    if (count <= 0) return;
+    if (rec0 >= 0) ensure_trace(rec0 + count);
+    if (exec_req_ != 0 && n_ >= 2 && !plan_.empty()) {
+        // snapshots of a recorded chunk stay resident until its entropies are
+        // taken; bound the chunk to ~8 GiB of snapshots
+        const std::size_t per = off_[n_] * sizeof(double2);
+        const int chunk = rec0 >= 0 ? (int)std::max<std::size_t>(1, std::min<std::size_t>(64, (8ull << 30) / per)) : 64;
+        for (int k0 = 0; k0 < count; k0 += chunk) {
+            const int c = std::min(chunk, count - k0);
+            run_exec(a + k0, b + k0, c, rec0 >= 0 ? rec0 + k0 : -1);
+        }
+        return;
+    }
+    for (int k = 0; k < count; ++k) step(a[k], b[k], rec0 >= 0 ? rec0 + k : -1);
+}
+
+void Tebd::set_executor(int groups) {
+    if (groups < -1) throw Error(-1, "executor groups must be >= -1");
+    sync();
+    exec_req_ = groups;
+}
+
+// Groups of `per_group` workgroups (enough for the widest block-Jacobi round),
+// as many groups as are co-resident (at most 64), a multiple of 8 when 8 or
+// more fit.
+void Tebd::exec_shape(int& groups, int& per_group, std::size_t& shm) const {
+    const int cmax = 2 * maxcap_;
+    const int nb = (cmax + kBlk - 1) / kBlk;
+    const int nbp = std::max(2, nb + (nb & 1));
+    per_group = nbp / 2;
+    const std::size_t svd_bytes = (std::size_t)2 * kBlk * (2 * maxcap_) * sizeof(double2);
+    const std::size_t theta_bytes = (std::size_t)2 * 2 * 16 * 2 * 17 * sizeof(double2);
+    shm = std::max(svd_bytes, theta_bytes);
+    int per_cu = 0, cus = 0, dev = 0;
+    TQ_HIP(hipGetDevice(&dev));
+    TQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_tebd_lockstep, kDistThreads, shm));
+    const long long resident = std::min<long long>((long long)per_cu * cus, kStatSlots);
+    long long g = resident / per_group;
+    const long long cap = exec_req_ > 0 ? exec_req_ : 64;
+    g = std::min(g, cap);
+    if (g >= 8) g -= g % 8;
+    if (g < 1) throw Error(-3, "lockstep executor: not enough co-resident workgroups");
+    groups = (int)g;
+}
+
+void Tebd::run_exec(const double* a, const double* b, int count, int rec0) {
+    TQ_HIP(hipStreamSynchronize(stream_));  // the previous run's buffers are free again
+    int G = 0, W = 0;
+    std::size_t shm = 0;
+    exec_shape(G, W, shm);
+    using cd = std::complex<double>;
+    struct M2 {
+        cd u[2][2];
+    };
+    const M2 eye{{{1.0, 0.0}, {0.0, 1.0}}};
+    auto mul2 = [](const M2& x, const M2& y) {
+        M2 r;
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) r.u[i][j] = x.u[i][0] * y.u[0][j] + x.u[i][1] * y.u[1][j];
+        return r;
+    };
+    auto to_d2 = [](cd z) { return make_double2(z.real(), z.imag()); };
+    auto from_d2 = [](double2 z) { return cd(z.x, z.y); };
+    // exp(-i dt/2 g), g = [[-hz, -A], [-A, hz]] (the closed form of k_single_layer)
+    auto half_layer = [&](double A, double B, int i) {
+        const double hz = B * model_.h[i] + (model_.bias.empty() ? 0.0 : A * model_.bias[i]);
+        const double r = std::sqrt(A * A + hz * hz);
+        M2 u = eye;
+        if (r > 0.0) {
+            const double sn = std::sin(0.5 * dt_ * r), cs = std::cos(0.5 * dt_ * r);
+            u.u[0][0] = cd(cs, sn * hz / r);
+            u.u[0][1] = u.u[1][0] = cd(0.0, sn * A / r);
+            u.u[1][1] = cd(cs, -sn * hz / r);
+        }
+        return u;
+    };
+    auto nbp_of = [&](int p) {  // tournament length of the widest matrix update p can see
+        const int c = std::min(2 * cap_[p], 2 * cap_[p + 2]);
+        const int nb = (c + kBlk - 1) / kBlk;
+        return std::max(2, nb + (nb & 1));
+    };
+    const Gate4 sw = make_swap_gate();
+    std::vector<M2> pending(n_, eye);
+    std::vector<int> free_at(n_, 0);     // first super-step at which the site may be updated
+    std::vector<int> load;               // updates per super-step
+    std::vector<ExecUpdate> ups;         // [super-step][G]
+    std::vector<int> rounds;
+    struct SnapAt {
+        int at;
+        ExecSnap s;
+    };
+    std::vector<SnapAt> snaps;
+    std::vector<int> last_at(n_, -1);  // super-step of the site's latest update
+    int nsteps = 0;
+    auto place = [&](int p, const Gate4& g) {
+        int t = std::max(free_at[p], free_at[p + 1]);
+        while (t < (int)load.size() && load[t] >= G) ++t;
+        if (t >= (int)load.size()) {
+            load.resize(t + 1, 0);
+            rounds.resize(t + 1, 0);
+            ExecUpdate idle{};
+            idle.site = -1;
+            ups.resize((std::size_t)(t + 1) * G, idle);
+        }
+        ExecUpdate& u = ups[(std::size_t)t * G + load[t]++];
+        u.site = p;
+        u.g = g;
+        rounds[t] = std::max(rounds[t], nbp_of(p) - 1);
+        free_at[p] = free_at[p + 1] = t + 1;
+        nsteps = std::max(nsteps, t + 1);
+        return t;
+    };
+    for (int k = 0; k < count; ++k) {
+        const double A = a[k], B = b[k];
+        std::vector<M2> u1(n_);
+        for (int i = 0; i < n_; ++i) {
+            u1[i] = half_layer(A, B, i);
+            pending[i] = mul2(u1[i], pending[i]);
+        }
+        const double sub = dt_ / ns_;
+        auto add_edge = [&](const std::pair<std::size_t, std::size_t>& span) {
+            for (std::size_t q = span.first; q < span.second; ++q) {
+                const Update& u = plan_[q];
+                const Gate4 g = u.edge < 0 ? sw : make_zz_gate(-B * model_.ew[u.edge] * sub);
+                const M2& P = pending[u.site];
+                const M2& Q = pending[u.site + 1];
+                Gate4 f;
+                for (int r = 0; r < 4; ++r)
+                    for (int c = 0; c < 4; ++c) {
+                        cd acc = 0.0;
+                        for (int s = 0; s < 4; ++s)
+                            acc += from_d2(g.g[r * 4 + s]) * P.u[s >> 1][c >> 1] * Q.u[s & 1][c & 1];
+                        f.g[r * 4 + c] = to_d2(acc);
+                    }
+                pending[u.site] = eye;
+                pending[u.site + 1] = eye;
+                const int t = place(u.site, f);
+                last_at[u.site] = last_at[u.site + 1] = t;
+            }
+        };
+        for (int s = 0; s < ns_; ++s) {
+            if (s % 2 == 0)
+                for (const auto& span : spans_) add_edge(span);
+            else
+                for (std::size_t e = spans_.size(); e-- > 0;) add_edge(spans_[e]);
+        }
+        for (int i = 0; i < n_; ++i) pending[i] = mul2(u1[i], pending[i]);
+        if (rec0 >= 0)
+            for (int i = 0; i < n_; ++i) {
+                // after the site's last update of this step (sites change only in
+                // the reconstruction phase), before any later update changes it
+                SnapAt x;
+                x.at = last_at[i] + 1;
+                x.s.site = i;
+                x.s.slot = k;
+                x.s.g[0] = to_d2(pending[i].u[0][0]);
+                x.s.g[1] = to_d2(pending[i].u[0][1]);
+                x.s.g[2] = to_d2(pending[i].u[1][0]);
+                x.s.g[3] = to_d2(pending[i].u[1][1]);
+                snaps.push_back(x);
+            }
+    }
+    // snapshots due after the last super-step get a trailing super-step of their own
+    int total_steps = nsteps;
+    for (const auto& x : snaps) total_steps = std::max(total_steps, x.at + 1);
+    if (total_steps > nsteps) {
+        ExecUpdate idle{};
+        idle.site = -1;
+        ups.resize((std::size_t)total_steps * G, idle);
+        rounds.resize(total_steps, 0);
+    }
+    std::stable_sort(snaps.begin(), snaps.end(), [](const SnapAt& x, const SnapAt& y) { return x.at < y.at; });
+    std::vector<int> snap_begin(total_steps + 1, 0);
+    std::vector<ExecSnap> snap_list(snaps.size());
+    for (std::size_t q = 0; q < snaps.size(); ++q) {
+        snap_list[q] = snaps[q].s;
+        ++snap_begin[snaps[q].at + 1];
+    }
+    for (int t = 0; t < total_steps; ++t) snap_begin[t + 1] += snap_begin[t];
+    // a super-step whose slots are all idle still runs one (empty) round
+    for (int t = 0; t < total_steps; ++t) rounds[t] = std::max(rounds[t], 1);
+
+    // device buffers (grown on demand, kept across runs)
+    const std::size_t mat_sz = (std::size_t)ld_ * ld_;
+    if (ex_mat_.n < G * mat_sz) {
+        ex_mat_.alloc(G * mat_sz);
+        ex_work_.alloc(G * mat_sz);
+        ex_sv_.alloc((std::size_t)G * ld_);
+        ex_perm_.alloc((std::size_t)G * ld_);
+        ex_flags_.alloc(2 * (std::size_t)G + 2);
+    }
+    if (rec0 >= 0 && ex_snap_.n < (std::size_t)count * off_[n_]) {
+        ex_snap_.alloc((std::size_t)count * off_[n_]);
+        ex_sdims_.alloc((std::size_t)count * (n_ + 1));
+    }
+    const std::size_t up_bytes = ups.size() * sizeof(ExecUpdate);
+    const std::size_t sn_bytes = std::max<std::size_t>(1, snap_list.size()) * sizeof(ExecSnap);
+    const std::size_t int_count = (std::size_t)total_steps + (total_steps + 1);
+    ex_host_.resize(up_bytes + sn_bytes + int_count * sizeof(int));
+    unsigned char* hb = ex_host_.data();
+    std::memcpy(hb, ups.data(), up_bytes);
+    if (!snap_list.empty()) std::memcpy(hb + up_bytes, snap_list.data(), snap_list.size() * sizeof(ExecSnap));
+    int* hi = reinterpret_cast<int*>(hb + up_bytes + sn_bytes);
+    std::memcpy(hi, rounds.data(), total_steps * sizeof(int));
+    std::memcpy(hi + total_steps, snap_begin.data(), (total_steps + 1) * sizeof(int));
+    if (ex_tasks_.n < ex_host_.size()) ex_tasks_.alloc(ex_host_.size());
+    TQ_HIP(hipMemcpy(ex_tasks_.p, hb, ex_host_.size(), hipMemcpyHostToDevice));
+
+    ExecArgs x{};
+    x.ups = reinterpret_cast<const ExecUpdate*>(ex_tasks_.p);
+    x.snaps = reinterpret_cast<const ExecSnap*>(ex_tasks_.p + up_bytes);
+    x.rounds = reinterpret_cast<const int*>(ex_tasks_.p + up_bytes + sn_bytes);
+    x.snap_begin = x.rounds + total_steps;
+    x.nsteps = total_steps;
+    x.G = G;
+    x.sites = sites_.p;
+    x.off = off_dev_.p;
+    x.cap = caps_dev_.p;
+    x.dims = dims_.p;
+    x.n = n_;
+    x.chi = chi_;
+    x.mat = ex_mat_.p;
+    x.work = ex_work_.p;
+    x.ld = ld_;
+    x.sv = ex_sv_.p;
+    x.perm = ex_perm_.p;
+    x.svd_flags = ex_flags_.p;
+    x.gflags = ex_flags_.p + 2 * G;
+    x.snap = ex_snap_.p;
+    x.snap_stride = off_[n_];
+    x.snap_dims = ex_sdims_.p;
+    x.trunc = trunc_.p;
+    x.stats = stats_.p;
+    void* args[] = {(void*)&x};
+    if (timing_) {
+        if (ev_next_ >= events_.size()) {
+            hipEvent_t e0, e1;
+            TQ_HIP(hipEventCreate(&e0));
+            TQ_HIP(hipEventCreate(&e1));
+            events_.push_back({e0, e1});
+        }
+        TQ_HIP(hipEventRecord(events_[ev_next_].first, stream_));
+    }
+    TQ_HIP(hipLaunchCooperativeKernel((const void*)k_tebd_lockstep, dim3((unsigned)(G * W)), dim3(kDistThreads), args,
+                                      shm, stream_));
+    if (timing_) TQ_HIP(hipEventRecord(events_[ev_next_++].second, stream_));
+    exec_supersteps_ += total_steps;
+    for (int t = 0; t < nsteps; ++t) exec_updates_ += load[t];
+    // fold the single-site gates still pending into the sites
+    std::vector<double2> g(4 * (std::size_t)n_);
+    for (int i = 0; i < n_; ++i) {
+        g[4 * i + 0] = to_d2(pending[i].u[0][0]);
+        g[4 * i + 1] = to_d2(pending[i].u[0][1]);
+        g[4 * i + 2] = to_d2(pending[i].u[1][0]);
+        g[4 * i + 3] = to_d2(pending[i].u[1][1]);
+    }
+    if (!ex_gates_.p) ex_gates_.alloc(4 * (std::size_t)n_);
+    TQ_HIP(hipMemcpyAsync(ex_gates_.p, g.data(), g.size() * sizeof(double2), hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(k_apply_site_gates, dim3(n_), dim3(256), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p,
+                       dims_.p, ex_gates_.p);
+    TQ_LAUNCH_CHECK();
+    // entropies of the recorded steps from their snapshots
+    if (rec0 >= 0)
+        for (int k = 0; k < count; ++k)
+            entropy_of(ex_snap_.p + (std::size_t)k * off_[n_], ex_sdims_.p + (std::size_t)k * (n_ + 1), rec0 + k);
+    TQ_HIP(hipStreamSynchronize(stream_));  // host staging buffers go out of scope
+}
+
+void Tebd::env_right(const double2* rin, int j, double2* rout, const double2* sites, const int* dims) {
+    if (!sites) sites = sites_.p;
+    if (!dims) dims = dims_.p;
     const int ldr = chi_max();
     const long long w1 = (long long)cap_[j] * 2 * cap_[j + 1];
-    hipLaunchKernelGGL(k_envr1, dim3((unsigned)grid_for(w1, 256)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
-                       caps_dev_.p, dims_.p, j, rin, ldr, envT_.p);
+    hipLaunchKernelGGL(k_envr1, dim3((unsigned)grid_for(w1, 256)), dim3(256), 0, stream_, sites, off_dev_.p,
+                       caps_dev_.p, dims, j, rin, ldr, envT_.p);
     TQ_LAUNCH_CHECK();
     const long long w2 = (long long)cap_[j] * cap_[j];
-    hipLaunchKernelGGL(k_envr2, dim3((unsigned)grid_for(w2, 256)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
-                       caps_dev_.p, dims_.p, j, envT_.p, rout, ldr);
+    hipLaunchKernelGGL(k_envr2, dim3((unsigned)grid_for(w2, 256)), dim3(256), 0, stream_, sites, off_dev_.p,
+                       caps_dev_.p, dims, j, envT_.p, rout, ldr);
     TQ_LAUNCH_CHECK();
 }
 
@@ -898,7 +1511,9 @@ void Tebd::env_left(const double2* lin, int j, bool z, double2* lout) {
 }
 
 // mps.cpp:355 -> single_site_entropy(0): right environment over sites n-1..1.
-void Tebd::entropy_sweep(int rec) {
+void Tebd::entropy_sweep(int rec) { entropy_of(sites_.p, dims_.p, rec); }
+
+void Tebd::entropy_of(const double2* sites, const int* dims, int rec) {
     ensure_trace(rec + 1);
     if (n_ < 2) {
         const double zero = 0.0;
@@ -911,10 +1526,10 @@ void Tebd::entropy_sweep(int rec) {
     hipLaunchKernelGGL(k_set_unit, dim3(1), dim3(1), 0, stream_, cur);
     TQ_LAUNCH_CHECK();
     for (int j = n_ - 1; j >= 1; --j) {
-        env_right(cur, j, nxt);
+        env_right(cur, j, nxt, sites, dims);
         std::swap(cur, nxt);
     }
-    hipLaunchKernelGGL(k_rho0, dim3(1), dim3(64), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p, cur,
+    hipLaunchKernelGGL(k_rho0, dim3(1), dim3(64), 0, stream_, sites, off_dev_.p, caps_dev_.p, dims, cur,
                        chi_max(), trace_.p, rec);
     TQ_LAUNCH_CHECK();
 }
@@ -946,24 +1561,31 @@ std::vector<double> Tebd::entropy(int count) {
 }
 
 SvdStats Tebd::svd_stats() {
-    unsigned long long h[4] = {0, 0, 0, 0};
-    TQ_HIP(hipMemcpyAsync(h, stats_.p, sizeof(h), hipMemcpyDeviceToHost, stream_));
+    std::vector<unsigned long long> h(4 * (std::size_t)kStatSlots);
+    TQ_HIP(hipMemcpyAsync(h.data(), stats_.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
     sync();
+    unsigned long long tot[4] = {0, 0, 0, 0};
+    for (int b = 0; b < kStatSlots; ++b)
+        for (int k = 0; k < 4; ++k) tot[k] += h[4 * (std::size_t)b + k];
     SvdStats st;
-    st.calls = (long long)h[0];
-    st.sweeps = (long long)h[1];
-    st.visited_rows = (double)h[2];
-    st.rotated_rows = (double)h[3];
+    st.calls = (long long)tot[0];
+    st.sweeps = (long long)tot[1];
+    st.visited_rows = (double)tot[2];
+    st.rotated_rows = (double)tot[3];
     st.svd_ms = timed_ms_;
     st.timed_launches = timed_n_;
+    st.supersteps = exec_supersteps_;
+    st.exec_updates = exec_updates_;
     return st;
 }
 
 double Tebd::max_trunc() {
-    double t = 0.0;
-    TQ_HIP(hipMemcpyAsync(&t, trunc_.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    std::vector<double> t(kStatSlots);
+    TQ_HIP(hipMemcpyAsync(t.data(), trunc_.p, t.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
     sync();
-    return t;
+    double m = 0.0;
+    for (double v : t) m = std::max(m, v);
+    return m;
 }
 
 std::vector<int> Tebd::bonds() {

@@ -26,6 +26,7 @@ struct SvdStats {
     double visited_rows = 0, rotated_rows = 0;
     double svd_ms = 0;
     long long timed_launches = 0;
+    long long supersteps = 0, exec_updates = 0;  // lockstep executor: super-steps run, updates placed
 };
 
 // One two-site update of the routed ZZ block.
@@ -45,6 +46,14 @@ public:
     void randomize(uint64_t seed, double decay);  // saturated random state
     // Enqueue one Trotter step (mps.cpp:309-356); entropy recorded at slot `rec` if >= 0.
     void step(double A, double B, int rec);
+    // Enqueue `count` steps with schedule values a[k], b[k]; when rec0 >= 0 the
+    // entropy of step k is recorded at slot rec0 + k. Runs the steps one launch
+    // sequence per update, or as one persistent launch when the executor is on.
+    void run(const double* a, const double* b, int count, int rec0);
+    // Lockstep executor: 0 = off, g > 0 = at most g groups of workgroups,
+    // -1 = as many groups as are co-resident (capped at 64).
+    void set_executor(int groups);
+    int executor_groups() const { return exec_req_; }
     void sync();
     std::vector<double> entropy(int count);      // recorded slots [0, count)
     double max_trunc();
@@ -66,10 +75,14 @@ public:
 
 private:
     void build_plan();
+    void run_exec(const double* a, const double* b, int count, int rec0);
+    void exec_shape(int& groups, int& per_group, std::size_t& shm) const;
     void two_site(int p, const Gate4& g);
     void entropy_sweep(int rec);
+    void entropy_of(const double2* sites, const int* dims, int rec);
     void env_left(const double2* lin, int site, bool z, double2* lout);
-    void env_right(const double2* rin, int site, double2* rout);
+    void env_right(const double2* rin, int site, double2* rout, const double2* sites = nullptr,
+                   const int* dims = nullptr);
     double2* site_ptr(int i) const { return sites_.p + off_[i]; }
     int chi_max() const { return maxcap_; }
 
@@ -100,11 +113,21 @@ private:
     long long timed_n_ = 0;
     int ld_ = 0;                   // leading dimension of mat_/work_
     bool force_dist_ = false;
+    int xcd_stride_ = 1;
     DevBuf<double> h_, bias_;
     DevBuf<double2> envA_, envB_, envT_;
     DevBuf<double> trace_;
     int trace_cap_ = 0;
     std::size_t svd_lds_ = 0;
+
+    // lockstep executor state
+    int exec_req_ = 0;
+    long long exec_supersteps_ = 0, exec_updates_ = 0;
+    DevBuf<unsigned char> ex_tasks_;   // schedule: updates, snapshots, rounds, snapshot offsets
+    DevBuf<double2> ex_mat_, ex_work_, ex_snap_, ex_gates_;
+    DevBuf<double> ex_sv_;
+    DevBuf<int> ex_perm_, ex_flags_, ex_sdims_;
+    std::vector<unsigned char> ex_host_;
 };
 
 }  // namespace tq

@@ -77,6 +77,10 @@ def lib() -> ctypes.CDLL:
         L.tessera_tebd_svd_stats.argtypes = [vp, vp, vp, vp, vp]
         L.tessera_tebd_set_timing.restype = ctypes.c_int
         L.tessera_tebd_set_timing.argtypes = [vp, ctypes.c_int]
+        L.tessera_tebd_set_executor.restype = ctypes.c_int
+        L.tessera_tebd_set_executor.argtypes = [vp, ctypes.c_int]
+        L.tessera_tebd_exec_stats.restype = ctypes.c_int
+        L.tessera_tebd_exec_stats.argtypes = [vp, vp, vp]
         L.tessera_tebd_updates_per_step.restype = _usz
         L.tessera_tebd_updates_per_step.argtypes = [vp]
         L.tessera_tebd_destroy.restype = None
@@ -297,6 +301,18 @@ class TebdSession:
 
     def set_timing(self, on: bool = True):
         _check(lib().tessera_tebd_set_timing(self._h, 1 if on else 0))
+
+    def set_executor(self, groups: int = -1):
+        """0: one launch sequence per update; g > 0: one lockstep cooperative launch
+        per run() with at most g concurrent groups of workgroups; -1: as many as fit."""
+        _check(lib().tessera_tebd_set_executor(self._h, int(groups)))
+
+    def exec_stats(self) -> dict:
+        """Lockstep executor totals: super-steps run and updates placed in them."""
+        ss = ctypes.c_int64()
+        up = ctypes.c_int64()
+        _check(lib().tessera_tebd_exec_stats(self._h, ctypes.addressof(ss), ctypes.addressof(up)))
+        return dict(supersteps=ss.value, updates=up.value)
 
     @property
     def updates_per_step(self) -> int:

@@ -33,10 +33,11 @@ sys.path.insert(0, str(ROOT))
 
 N_SPINS, RANGE, CHI, DT, SCHED_STEPS, SEED = 128, 4, 256, 0.04, 1000, 3
 FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
 # One saturated chi=256 update of the upstream engine runs its 1024 x 1024 real
 # Jacobi eigensolver for this many sweeps (measured once with the bit-exact CPU
 # port on this matrix; the per-sweep time is re-measured on the benchmark host).
-REF_SWEEPS_CHI256 = None
+REF_SWEEPS_CHI256 = 25
 
 
 def parse():
@@ -47,9 +48,17 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--n", type=int, default=N_SPINS)
     ap.add_argument("--chi", type=int, default=CHI)
-    ap.add_argument("--executor", type=int, default=-1,
-                    help="persistent executor groups (-1: as many as fit, 0: one launch sequence per update)")
+    ap.add_argument("--executor", type=int, default=0,
+                    help="lockstep executor groups (0: one launch sequence per update, -1: as many as fit)")
+    ap.add_argument("--workload", choices=("tebd", "sv"), default="tebd",
+                    help="tebd: the headline metric; sv: the n=30 state-vector sweep (HBM roofline)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (PMC byte-counter calibration) and exit")
     return ap.parse_args()
+
+
+def progress(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(chi: int):
@@ -70,12 +79,75 @@ def cpu_baseline(chi: int):
     return info
 
 
+def copy_calibration(mib: int):
+    """A plain device copy of known size: the PMC byte counters of this run,
+    divided by 2 * mib MiB, give the counter scale used for `traffic`."""
+    import torch
+
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
+
+
+SV_N, SV_SEED, SV_DT, SV_STEPS = 30, 2, 0.02, 1000
+
+
+def sv_bench(args):
+    """n=30 random 3-regular TFIM (J ~ N(0,1), h ~ N(0,0.25)), linear schedule
+    T=20, dt=0.02: per-step time of the phase + X-rotation sweep from the
+    difference of two anneal lengths (allocation, initial state and readout
+    cancel)."""
+    from tessera_amd import instances as I
+    from tessera_amd import qa
+
+    inst = I.config2(SV_SEED, SV_STEPS)
+    n = inst.model.n
+    w, k = max(args.warmup, 1), args.steps
+    full = inst.sched
+
+    def timed(steps):
+        sched = I.Schedule(full.a[:steps], full.b[:steps])
+        t0 = time.perf_counter()
+        qa.anneal_statevector(inst.model, sched, SV_DT)
+        return time.perf_counter() - t0
+
+    timed(1)  # code objects, first allocation
+    t_w = timed(w)
+    progress(f"sv warm run of {w} steps: {t_w:.3f}s")
+    t_wk = timed(w + k)
+    progress(f"sv run of {w + k} steps: {t_wk:.3f}s")
+    per_step = (t_wk - t_w) / k
+    algo = 32.0 * (1 << n)  # read + write every complex128 amplitude once per step
+    achieved = algo / per_step / 1e9
+    out = {
+        "metric": "state-vector Trotter steps/s @n=30 (HBM GB/s of the phase + X sweep)",
+        "value": 1.0 / per_step, "unit": "Trotter steps/s", "n_gpus": 1, "steps": k, "warmup": w,
+        "ms_per_step": per_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 (fp64)", "data": "synthetic: seeded random 3-regular graph",
+        "config": {"workload": f"exact state vector n={n}, 2^{n} amplitudes, dt={SV_DT}"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sv_pass (2 passes per step at n=30, algorithmic bytes = 32 * 2^n per step)"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding is a design in DESIGN.md",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "sv":
+        sv_bench(args)
+        return
     import numpy as np
 
     from tessera_amd import instances as I
@@ -90,15 +162,26 @@ def main():
     ups = sess.updates_per_step
     a = sched.a[k0:k0 + args.warmup + args.steps]
     b = sched.b[k0:k0 + args.warmup + args.steps]
+    progress(f"n={args.n} chi={args.chi} updates/step={ups} executor={args.executor}")
     if args.warmup:
+        tw = time.perf_counter()
         sess.run(a[:args.warmup], b[:args.warmup])
         sess.sync()
+        progress(f"warmup {args.warmup} step(s): {time.perf_counter() - tw:.2f}s")
     sess.set_timing(True)
     st0 = sess.svd_stats()
+    ex0 = sess.exec_stats()
     t0 = time.perf_counter()
-    sess.run(a[args.warmup:], b[args.warmup:])
+    if args.executor:
+        sess.run(a[args.warmup:], b[args.warmup:])  # one launch: the steps overlap
+    else:
+        for k in range(args.steps):  # a progress line per step
+            sess.run(a[args.warmup + k:args.warmup + k + 1], b[args.warmup + k:args.warmup + k + 1])
+            sess.sync()
+            progress(f"step {k + 1}/{args.steps}: {time.perf_counter() - t0:.2f}s")
     ent, trunc, bonds = sess.sync()
     t1 = time.perf_counter()
+    progress(f"timed {args.steps} step(s): {t1 - t0:.2f}s")
     st1 = sess.svd_stats()
     wall = t1 - t0
     updates = ups * args.steps
@@ -109,8 +192,11 @@ def main():
     flops = 16.0 * vis + 18.0 * rot
     achieved = flops / (svd_ms * 1e-3) / 1e12 if svd_ms > 0 else 0.0
     if args.executor:
-        kernel = "k_tebd_exec (persistent: theta, block-Jacobi SVD, reconstruction, entropy; Jacobi flops counted)"
-        timing = {"kernel_ms": svd_ms, "kernel_share_of_wall": svd_ms * 1e-3 / wall}
+        ex1 = sess.exec_stats()
+        ss = ex1["supersteps"] - ex0["supersteps"]
+        kernel = "k_tebd_lockstep (theta, lockstep block-Jacobi SVDs, reconstruction; Jacobi flops counted)"
+        timing = {"kernel_ms": svd_ms, "kernel_share_of_wall": svd_ms * 1e-3 / wall, "supersteps": ss,
+                  "updates_per_superstep": (ex1["updates"] - ex0["updates"]) / max(ss, 1)}
     else:
         kernel = "one-sided Jacobi SVD (k_svd_dist / k_svd)"
         timing = {"svd_ms_per_update": svd_ms / max(calls, 1), "svd_share_of_wall": svd_ms * 1e-3 / wall}

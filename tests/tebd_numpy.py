@@ -6,13 +6,25 @@ import math
 import numpy as np
 
 
-def tebd_numpy(model, sched, dt, chi):
+def _entropy0(sites):
+    """Qubit-0 entropy (log2) of the MPS contracted to a dense vector (small n)."""
+    psi = sites[0].reshape(2, -1)
+    for t in sites[1:]:
+        psi = np.einsum("xa,asb->xsb", psi, t).reshape(psi.shape[0] * 2, -1)
+    psi = psi.reshape(2, -1)  # site 0 is the leading index
+    rho = psi @ psi.conj().T
+    ev = np.clip(np.linalg.eigvalsh(rho / np.trace(rho).real), 0, 1)
+    return -sum(l * math.log2(l) for l in ev if l > 1e-12)
+
+
+def tebd_numpy(model, sched, dt, chi, want_entropy=False):
     n = model.n
     sites = [np.zeros((1, 2, 1), complex) for _ in range(n)]
     for t in sites:
         t[0, 0, 0] = t[0, 1, 0] = 1 / math.sqrt(2)
     trunc = 0.0
     bonds = []
+    ents = []
     swap = np.zeros((4, 4))
     swap[0, 0] = swap[1, 2] = swap[2, 1] = swap[3, 3] = 1
 
@@ -53,4 +65,8 @@ def tebd_numpy(model, sched, dt, chi):
                 gate2(k, swap)
         layer(A, B)
         bonds.append(max(t.shape[2] for t in sites))
+        if want_entropy:
+            ents.append(_entropy0(sites))
+    if want_entropy:
+        return trunc, bonds, np.array(ents)
     return trunc, bonds

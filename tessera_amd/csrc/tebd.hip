@@ -146,10 +146,22 @@ __device__ inline void pair_dots(const double2 (&x)[kRowsPerLane], const double2
     gi = wave_sum(gi);
 }
 
+// Columns whose norm^2 is at most floor2 hold rounding noise (theta = A C has
+// rank <= the inner bond, so up to half its columns converge to noise, whose
+// direction no relative test can settle): pairs with such a column are not
+// rotated. floor2 = min(kNoise^2 |theta|_F^2, kNoiseAbs^2): a skipped column
+// keeps S <= 5e-15 and so always falls under the keep rule's S < 1e-14 cut (a
+// kept column that was never orthogonalised would be an O(1) error once
+// normalised, which is why the floor is capped in absolute terms).
+constexpr double kNoise = 7.105427357601002e-15;  // 2^-47
+constexpr double kNoiseAbs = 5e-15;
+__device__ inline double noise_floor2(double fro2) { return fmin(kNoise * kNoise * fro2, kNoiseAbs * kNoiseAbs); }
+
 // One Jacobi rotation of columns x, y (lane-held rows); returns true if applied.
-__device__ inline bool pair_rotate(double2 (&x)[kRowsPerLane], double2 (&y)[kRowsPerLane], int m) {
+__device__ inline bool pair_rotate(double2 (&x)[kRowsPerLane], double2 (&y)[kRowsPerLane], int m, double floor2) {
     double al, be, gr, gi;
     pair_dots(x, y, al, be, gr, gi);
+    if (al <= floor2 || be <= floor2) return false;
     const double gabs = sqrt(gr * gr + gi * gi);
     if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * sqrt(al * be)) return false;
     const double2 ph = make_double2(gr / gabs, -gi / gabs);  // conj(gamma)/|gamma|
@@ -208,6 +220,22 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
         s_vis = 0;
         s_rot = 0;
     }
+    // |theta|_F^2 for the noise floor of pair_rotate
+    __shared__ double s_fro[kSvdThreads / 64];
+    {
+        double a = 0.0;
+        for (int idx = tid; idx < m * c; idx += blockDim.x) {
+            const int col = idx / m, row = idx % m;
+            const double2 v = W[(std::size_t)col * lw + row];
+            a += v.x * v.x + v.y * v.y;
+        }
+        a = wave_sum(a);
+        if (lane == 0) s_fro[wave] = a;
+        __syncthreads();
+    }
+    double fro = 0.0;
+    for (int w = 0; w < nwaves; ++w) fro += s_fro[w];
+    const double floor2 = noise_floor2(fro);
     const int cp = c + (c & 1);
     int sweeps_done = 0;
     unsigned long long visited = 0, rotated = 0;  // pair visits / rotations (wave-local, lane 0 counts)
@@ -226,7 +254,7 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
                 load_col(xa, x, m, lane);
                 load_col(ya, y, m, lane);
                 ++visited;
-                if (pair_rotate(xa, ya, m)) {
+                if (pair_rotate(xa, ya, m, floor2)) {
                     ++rotated;
                     if (lane == 0) s_flag = 1;
                     store_col(xa, x, m, lane);
@@ -302,8 +330,13 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
 // Columns are split into blocks of 8; each workgroup holds two blocks in LDS and
 // the blocks meet in round-robin tournament order, one grid barrier per outer
 // step. At the first outer step of a sweep every block also rotates its own
-// column pairs, so each sweep visits every column pair exactly once. Rotation
-// formulas, FP64 angles and the convergence test are those of k_svd.
+// column pairs, so each sweep visits every column pair exactly once. In the
+// cross phase wave w keeps column w of the first block in registers and meets
+// column (w + s) mod 8 of the second block in step s; column norms are carried
+// across rotations (|x'|^2 = |x|^2 - t|g|, |y'|^2 = |y|^2 + t|g|), recomputed at
+// every load and whenever a norm falls below 1/64 of its previous value, so a
+// cross pair costs one two-value wave reduction. Rotation formulas, FP64 angles
+// and the convergence test are those of k_svd, including the noise floor below.
 constexpr int kBlk = 8;
 constexpr int kDistThreads = 512;  // 8 waves: one column pair per wave
 constexpr int kStatSlots = 1024;   // per-workgroup accounting slots (largest grid)
@@ -312,9 +345,53 @@ constexpr int kStatSlots = 1024;   // per-workgroup accounting slots (largest gr
 struct SvdShared {
     double nrm[kMaxCols];
     int perm[kMaxCols];
+    double bn[kBlk];  // norms^2 of the LDS block's columns
     int rot;
     unsigned long long vis, rots;
 };
+
+// |x|^2 over the lane-held rows, summed over the wave.
+__device__ inline double col_norm2(const double2 (&x)[kRowsPerLane]) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) a += x[q].x * x[q].x + x[q].y * x[q].y;
+    return wave_sum(a);
+}
+
+// pair_rotate with carried norms^2 al (x) and be (y): only g = x^H y is
+// reduced; the norms are updated (recomputed when they shrink by 64x or more).
+__device__ inline bool pair_rotate_carried(double2 (&x)[kRowsPerLane], double2 (&y)[kRowsPerLane], int m,
+                                           double& al, double& be, double floor2) {
+    if (al <= floor2 || be <= floor2) return false;
+    double gr = 0.0, gi = 0.0;
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) {
+        const double2 u = x[q], v = y[q];
+        gr += u.x * v.x + u.y * v.y;  // conj(u) v
+        gi += u.x * v.y - u.y * v.x;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        gr += __shfl_xor(gr, off, 64);
+        gi += __shfl_xor(gi, off, 64);
+    }
+    const double gabs = sqrt(gr * gr + gi * gi);
+    if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * sqrt(al * be)) return false;
+    const double2 ph = make_double2(gr / gabs, -gi / gabs);  // conj(gamma)/|gamma|
+    const double zeta = (be - al) / (2.0 * gabs);
+    const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) {
+        const double2 u = x[q], v = cmul(y[q], ph);
+        x[q] = make_double2(cs * u.x - sn * v.x, cs * u.y - sn * v.y);
+        y[q] = make_double2(sn * u.x + cs * v.x, sn * u.y + cs * v.y);
+    }
+    const double al1 = al - t * gabs, be1 = be + t * gabs;
+    al = (al1 < al * (1.0 / 64)) ? col_norm2(x) : al1;
+    be = (be1 < be * (1.0 / 64)) ? col_norm2(y) : be1;
+    return true;
+}
 
 // How a block-Jacobi SVD is synchronised with others in the same launch. Alone:
 // every sync() is its own, rounds and sweeps follow its own matrix. Lockstep:
@@ -349,15 +426,27 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     const int nbp = nb + (nb & 1) < 2 ? 2 : nb + (nb & 1);
     const int P = nbp / 2;
     const int rounds = mode.rounds >= 0 ? mode.rounds : nbp - 1;
+    double* fro = reinterpret_cast<double*>(flags + 2);  // |theta|_F^2 accumulator
     if (me == 0 && tid == 0) {
         flags[0] = 0;
         flags[1] = 0;
+        *fro = 0.0;
     }
     if (tid == 0) {
         sh.vis = 0;
         sh.rots = 0;
     }
     sync();
+    if (mode.active)
+        for (int col = me * (kDistThreads / 64) + wave; col < c; col += nworkers * (kDistThreads / 64)) {
+            const double2* x = Wg + (std::size_t)col * ld;
+            double a = 0.0;
+            for (int row = lane; row < m; row += 64) a += x[row].x * x[row].x + x[row].y * x[row].y;
+            a = wave_sum(a);
+            if (lane == 0) atomicAdd(fro, a);
+        }
+    sync();
+    const double floor2 = noise_floor2(__hip_atomic_load(fro, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     bool working = mode.active;
     int sweeps_done = 0;
     unsigned long long nvis = 0, nrot = 0;
@@ -391,7 +480,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                             double2* py = lds + (std::size_t)(half * kBlk + j) * m;
                             load_col(xa, px, m, lane);
                             load_col(ya, py, m, lane);
-                            if (pair_rotate(xa, ya, m)) {
+                            if (pair_rotate(xa, ya, m, floor2)) {
                                 rotated = true;
                                 ++nrot;
                                 store_col(xa, px, m, lane);
@@ -402,26 +491,36 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                         __syncthreads();
                     }
                 }
-                // cross pairs: wave w keeps member w of block bi in registers and
-                // meets member (w + s) & 7 of block bj in round s
+                // cross pairs with carried norms
                 const bool mine = bi * kBlk + wave < c;
-                if (mine) load_col(xa, lds + (std::size_t)wave * m, m, lane);
+                double al = 0.0;
+                if (mine) {
+                    load_col(xa, lds + (std::size_t)wave * m, m, lane);
+                    al = col_norm2(xa);
+                }
+                if (bj * kBlk + wave < c) {
+                    load_col(ya, lds + (std::size_t)(kBlk + wave) * m, m, lane);
+                    const double b2 = col_norm2(ya);
+                    if (lane == 0) sh.bn[wave] = b2;
+                }
+                __syncthreads();
                 for (int s = 0; s < kBlk; ++s) {
                     const int j = (wave + s) & (kBlk - 1);
                     if (mine && bj * kBlk + j < c) {
                         double2* py = lds + (std::size_t)(kBlk + j) * m;
                         load_col(ya, py, m, lane);
-                        if (pair_rotate(xa, ya, m)) {
+                        double be = sh.bn[j];
+                        if (pair_rotate_carried(xa, ya, m, al, be, floor2)) {
                             rotated = true;
                             ++nrot;
                             store_col(ya, py, m, lane);
+                            if (lane == 0) sh.bn[j] = be;
                         }
                         ++nvis;
                     }
                     __syncthreads();
                 }
                 if (mine) store_col(xa, lds + (std::size_t)wave * m, m, lane);
-                __syncthreads();
                 if (rotated && lane == 0) sh.rot = 1;
                 __syncthreads();
                 if (tid == 0 && sh.rot) {
@@ -812,21 +911,22 @@ __global__ void k_randomize(double2* sites, const std::size_t* off, const int* c
 }
 
 // theta of update p by 16x16 tiles (the computation of k_theta); each workgroup
-// computes two tiles at a time (thread halves), tiles dealt round-robin over the
-// group's workgroups.
+// computes blockDim/256 tiles at a time, tiles dealt round-robin over the group's
+// workgroups.
 __device__ void theta_tiles(const double2* __restrict__ sites, const std::size_t* off, const int* cap,
                             const int* dims, int p, const Gate4& g, double2* __restrict__ M, double2* __restrict__ W,
                             int ld, int rank, int nwg, double2* tl) {
     const int Dl = dims[p], mid = dims[p + 1], Dr = dims[p + 2];
     const int sA = cap[p + 1], sC = cap[p + 2];
     const std::size_t oA = off[p], oC = off[p + 1];
-    const int half = threadIdx.x >> 8, t16 = threadIdx.x & 255, tx = t16 & 15, ty = t16 >> 4;
-    double2* As = tl + half * (2 * 16 * 2 * 17);  // [16][2][17]
+    const int slot = threadIdx.x >> 8, nslot = blockDim.x >> 8;
+    const int t16 = threadIdx.x & 255, tx = t16 & 15, ty = t16 >> 4;
+    double2* As = tl + slot * (2 * 16 * 2 * 17);  // [16][2][17]
     double2* Cs = As + 16 * 2 * 17;
     const int nta = (Dl + 15) / 16, ntb = (Dr + 15) / 16, total = nta * ntb;
     const bool tall = Dr <= Dl;
-    for (int base = rank * 2; base < total; base += nwg * 2) {
-        const int tile = base + half;
+    for (int base = rank * nslot; base < total; base += nwg * nslot) {
+        const int tile = base + slot;
         const bool active = tile < total;
         const int ta = active ? tile / ntb : 0, tb = active ? tile % ntb : 0;
         const int a = ta * 16 + ty, b = tb * 16 + tx;
@@ -907,7 +1007,7 @@ struct ExecArgs {
     int ld;
     double* sv;
     int* perm;
-    int* svd_flags;     // [G][2]
+    int* svd_flags;     // [G][4]: rotation flags and a double accumulator per group
     int* gflags;        // [2]
     double2* snap;
     std::size_t snap_stride;
@@ -928,7 +1028,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
     double2* W = a.work + grp * mat_sz;
     double* sv = a.sv + (std::size_t)grp * a.ld;
     int* perm = a.perm + (std::size_t)grp * a.ld;
-    int* sflags = a.svd_flags + 2 * grp;
+    int* sflags = a.svd_flags + 4 * grp;
     const long long gid = (long long)rank * blockDim.x + tid, gstride = (long long)nwg * blockDim.x;
     const long long tid_all = (long long)blockIdx.x * blockDim.x + tid;
     const long long all = (long long)gridDim.x * blockDim.x;
@@ -1029,7 +1129,7 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     maxcap_ = maxcap;
     const int ld = 2 * maxcap;
     ld_ = ld;
-    flags_.alloc(2);
+    flags_.alloc(4);  // two rotation flags + a double accumulator
     stats_.alloc(4 * (std::size_t)kStatSlots);
     TQ_HIP(hipMemset(stats_.p, 0, 4 * (std::size_t)kStatSlots * sizeof(unsigned long long)));
     mat_.alloc((std::size_t)ld * ld);
@@ -1246,7 +1346,7 @@ void Tebd::exec_shape(int& groups, int& per_group, std::size_t& shm) const {
     const int nbp = std::max(2, nb + (nb & 1));
     per_group = nbp / 2;
     const std::size_t svd_bytes = (std::size_t)2 * kBlk * (2 * maxcap_) * sizeof(double2);
-    const std::size_t theta_bytes = (std::size_t)2 * 2 * 16 * 2 * 17 * sizeof(double2);
+    const std::size_t theta_bytes = (std::size_t)(kDistThreads / 256) * 2 * 16 * 2 * 17 * sizeof(double2);
     shm = std::max(svd_bytes, theta_bytes);
     int per_cu = 0, cus = 0, dev = 0;
     TQ_HIP(hipGetDevice(&dev));
@@ -1405,7 +1505,7 @@ void Tebd::run_exec(const double* a, const double* b, int count, int rec0) {
         ex_work_.alloc(G * mat_sz);
         ex_sv_.alloc((std::size_t)G * ld_);
         ex_perm_.alloc((std::size_t)G * ld_);
-        ex_flags_.alloc(2 * (std::size_t)G + 2);
+        ex_flags_.alloc(4 * (std::size_t)G + 2);
     }
     if (rec0 >= 0 && ex_snap_.n < (std::size_t)count * off_[n_]) {
         ex_snap_.alloc((std::size_t)count * off_[n_]);
@@ -1443,7 +1543,7 @@ void Tebd::run_exec(const double* a, const double* b, int count, int rec0) {
     x.sv = ex_sv_.p;
     x.perm = ex_perm_.p;
     x.svd_flags = ex_flags_.p;
-    x.gflags = ex_flags_.p + 2 * G;
+    x.gflags = ex_flags_.p + 4 * G;
     x.snap = ex_snap_.p;
     x.snap_stride = off_[n_];
     x.snap_dims = ex_sdims_.p;

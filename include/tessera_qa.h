@@ -116,8 +116,11 @@ int tessera_tebd_create(const double* h, size_t n, const size_t* ei, const size_
                         tessera_tebd** out);
 /* |+>^n product state (mps.cpp:9-17). */
 int tessera_tebd_reset(tessera_tebd* t);
-/* Random state with every bond at its capacity min(chi, 2^b, 2^(n-b)) and
- * bond weights exp(-k/decay); the steady state of a saturated run. */
+/* Benchmark state: random site tensors with every bond at its capacity
+ * min(chi, 2^b, 2^(n-b)), then one left-to-right sweep of identity two-site
+ * updates that leaves every site but the last an isometry and gives bond b the
+ * weights w_k ~ exp(-k/decay) (sum w_k^2 = 1): a saturated MPS whose Trotter
+ * steps truncate little, like the saturated phase of a real anneal. */
 int tessera_tebd_randomize(tessera_tebd* t, uint64_t seed, double decay);
 /* Enqueue `count` Trotter steps with schedule values a[k], b[k] (no host sync);
  * qubit-0 entropy of each step is recorded on the device when record != 0. */
@@ -131,6 +134,11 @@ int tessera_tebd_readout(tessera_tebd* t, int8_t* out_s, double* out_scalars);
 /* Dense amplitudes (n <= 24): out[2*x], out[2*x+1] = re, im of psi(x), where bit
  * i of x is the physical index of site i (0 <-> spin +1). */
 int tessera_tebd_dense(tessera_tebd* t, double* out);
+/* Diagnostic: theta of sites (p, p+1) under the identity gate, the matrix the
+ * next update at p factorises: out_rows x out_cols (2Dl x 2Dr) complex values,
+ * row-major, interleaved re/im; `cap` = complex entries available in out (out
+ * may be NULL to query the shape). */
+int tessera_tebd_theta(tessera_tebd* t, size_t p, double* out, size_t cap, int32_t* out_rows, int32_t* out_cols);
 /* Diagnostics since creation: SVDs run, Jacobi sweeps they took, out_rows[2] =
  * column-pair visits and rotations weighted by column length (pair-rows; the
  * one-sided Jacobi does 16 flops per visited pair-row and 18 more per rotated
@@ -150,6 +158,14 @@ int tessera_tebd_set_timing(tessera_tebd* t, int on);
  * per-step entropy is taken from site snapshots. Results equal the launch-per-
  * update path up to rounding. */
 int tessera_tebd_set_executor(tessera_tebd* t, int groups);
+/* Batched anneal (no upstream counterpart; the batched scheduler): advance
+ * count sessions together, session i by steps[i] Trotter steps with schedule
+ * values a[i][k], b[i][k], in one cooperative launch whose super-steps take
+ * independent two-site updates from all sessions (each session's gate order is
+ * kept). groups as for tessera_tebd_set_executor (-1: as many as fit). No
+ * entropy is recorded; read each session out with tessera_tebd_readout. */
+int tessera_tebd_batch_run(tessera_tebd* const* sessions, size_t count, const double* const* a,
+                           const double* const* b, const size_t* steps, int groups);
 /* Lockstep executor totals since creation: super-steps run and two-site
  * updates placed in them (updates / (super-steps * groups) = slot fill). */
 int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* out_updates);

@@ -301,6 +301,24 @@ int tessera_tebd_dense(tessera_tebd* t, double* out) {
     }
 }
 
+int tessera_tebd_theta(tessera_tebd* t, size_t p, double* out, size_t cap, int32_t* out_rows, int32_t* out_cols) {
+    try {
+        if (!t || !out_rows || !out_cols) throw Error(TESSERA_ERR_ARG, "null argument");
+        int rows = 0, cols = 0;
+        const std::vector<double> th = t->eng->theta((int)p, rows, cols);
+        *out_rows = rows;
+        *out_cols = cols;
+        if (out) {
+            if (cap < th.size() / 2) throw Error(TESSERA_ERR_ARG, "out too small for theta");
+            std::memcpy(out, th.data(), th.size() * sizeof(double));
+        }
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
 int tessera_tebd_svd_stats(tessera_tebd* t, int64_t* out_calls, int64_t* out_sweeps, double* out_rows,
                            double* out_ms) {
     try {
@@ -339,6 +357,38 @@ int tessera_tebd_set_executor(tessera_tebd* t, int groups) {
         if (!t) throw Error(TESSERA_ERR_ARG, "null session");
         if (groups < -1) throw Error(TESSERA_ERR_ARG, "groups must be >= -1");
         t->eng->set_executor(groups);
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_batch_run(tessera_tebd* const* sessions, size_t count, const double* const* a,
+                           const double* const* b, const size_t* steps, int groups) {
+    try {
+        if (!sessions || (count && (!a || !b || !steps))) throw Error(TESSERA_ERR_ARG, "null argument");
+        if (groups < -1) throw Error(TESSERA_ERR_ARG, "groups must be >= -1");
+        std::vector<Tebd*> ts;
+        std::vector<const double*> as, bs;
+        std::vector<int> cs;
+        for (size_t i = 0; i < count; ++i) {
+            if (!sessions[i]) throw Error(TESSERA_ERR_ARG, "null session");
+            if (steps[i] && (!a[i] || !b[i])) throw Error(TESSERA_ERR_ARG, "null schedule");
+            for (size_t k = 0; k < steps[i]; ++k)
+                if (!std::isfinite(a[i][k]) || !std::isfinite(b[i][k])) throw Error(TESSERA_ERR_NONFINITE, "non-finite schedule");
+            if (sessions[i]->eng->n() < 2) {  // nothing two-site to batch: plain path
+                sessions[i]->eng->run(a[i], b[i], (int)steps[i], -1);
+                continue;
+            }
+            for (Tebd* t : ts)
+                if (t == sessions[i]->eng.get()) throw Error(TESSERA_ERR_ARG, "session listed twice");
+            ts.push_back(sessions[i]->eng.get());
+            as.push_back(a[i]);
+            bs.push_back(b[i]);
+            cs.push_back((int)steps[i]);
+        }
+        Tebd::run_lockstep(ts, as, bs, cs, -1, groups);
         return TESSERA_OK;
     } catch (const std::exception& e) {
         set_last_error(e.what());

@@ -348,7 +348,141 @@ struct SvdShared {
     double bn[kBlk];  // norms^2 of the LDS block's columns
     int rot;
     unsigned long long vis, rots;
+    double2 red[kDistThreads / 64][16];  // LQ: per-wave partial row dots
+    double2 hh[2];                        // LQ: tau, 1/(alpha - beta)
+    double beta;
 };
+
+// ---- LQ preconditioning of the block-Jacobi SVD ----------------------------
+// W (m x c, m >= c, column-major) <- W Q = L, lower trapezoidal, by Householder
+// reflectors applied from the right (reflector k zeroes row k beyond column k).
+// L has the left singular vectors and singular values of W, and the one-sided
+// Jacobi converges on L's columns in about half the sweeps it needs on W's
+// (the rows of a QR/LQ factor are graded; on the thetas of a saturated run 10-14
+// sweeps instead of 22-43 in a CPU study). Rows go in tiles of 16; tile t
+// belongs to worker t mod nworkers. Panel q (rows 16q..16q+15) is factorised by
+// its owner in LDS, then every worker applies the panel's 16 reflectors to its
+// later tiles: one grid barrier per panel. Reflector k is kept as column k of
+// `aux` (entries at rows >= k, v_k = 1) with tau in taus[k].
+// Tile in LDS: element (row rr, column r0 + jj) at lds[jj * 16 + rr]; thread
+// (rr = tid & 15, g = tid >> 4) owns row rr, columns jj = g, g + 32, ...
+__device__ __forceinline__ void lq_load_tile(const double2* __restrict__ Wg, int ld, double2* lds, int t, int r0,
+                                             int n, int m) {
+    for (int idx = threadIdx.x; idx < n * 16; idx += blockDim.x) {
+        const int jj = idx >> 4, rr = idx & 15, row = t * 16 + rr;
+        lds[idx] = row < m ? Wg[(std::size_t)(r0 + jj) * ld + row] : make_double2(0, 0);
+    }
+    __syncthreads();
+}
+__device__ __forceinline__ void lq_store_tile(double2* __restrict__ Wg, int ld, const double2* lds, int t, int r0,
+                                              int n, int m) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < n * 16; idx += blockDim.x) {
+        const int jj = idx >> 4, rr = idx & 15, row = t * 16 + rr;
+        if (row < m) Wg[(std::size_t)(r0 + jj) * ld + row] = lds[idx];
+    }
+}
+// sum over the 32 column groups of each tile row: in-wave over the 4 lanes of
+// a row, then over the waves through LDS; returns the row's total to all its threads.
+__device__ __forceinline__ double2 lq_row_sum(double2 v, SvdShared& sh) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, rr = tid & 15;
+    v.x += __shfl_xor(v.x, 16, 64);
+    v.y += __shfl_xor(v.y, 16, 64);
+    v.x += __shfl_xor(v.x, 32, 64);
+    v.y += __shfl_xor(v.y, 32, 64);
+    if (lane < 16) sh.red[wave][rr] = v;
+    __syncthreads();
+    double2 tot = make_double2(0, 0);
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot = cadd(tot, sh.red[w][rr]);
+    __syncthreads();
+    return tot;
+}
+// tile rows rr > rmin: row <- row (I - tau v v^H), reflector from column i0 on;
+// reflector entry jj at v[jj * vs].
+__device__ __forceinline__ void lq_apply(double2* lds, int n, int i0, const double2* v, int vs, double2 tau,
+                                         int rmin, SvdShared& sh) {
+    const int rr = threadIdx.x & 15, g = threadIdx.x >> 4, ng = blockDim.x >> 4;
+    const int j0 = i0 + ((g - i0 % ng + ng) % ng);  // first jj >= i0 with jj = g (mod ng)
+    double2 w = make_double2(0, 0);
+    for (int jj = j0; jj < n; jj += ng) w = cadd(w, cmul(lds[jj * 16 + rr], v[(std::size_t)jj * vs]));
+    w = lq_row_sum(w, sh);
+    if (rr > rmin) {
+        const double2 tw = cmul(tau, w);
+        for (int jj = j0; jj < n; jj += ng)
+            lds[jj * 16 + rr] = csub(lds[jj * 16 + rr], cmulc(v[(std::size_t)jj * vs], tw));  // - tau w conj(v)
+    }
+    __syncthreads();
+}
+
+template <class Sync>
+__device__ void lq_precondition(double2* __restrict__ Wg, int ld, int m, int c, double2* __restrict__ aux,
+                                double2* __restrict__ taus, int me, int nworkers, Sync& sync, double2* lds,
+                                SvdShared& sh, int panels, bool active) {
+    const int tid = threadIdx.x, rr = tid & 15, g = tid >> 4, ng = blockDim.x >> 4;
+    const int ntiles = (m + 15) / 16;
+    for (int q = 0; q < panels; ++q) {
+        const int r0 = q * 16;
+        const bool live = active && r0 < c;
+        const int n = c - r0, nb = min(16, c - r0);
+        if (live && q % nworkers == me) {  // factorise panel q
+            lq_load_tile(Wg, ld, lds, q, r0, n, m);
+            for (int i = 0; i < nb; ++i) {
+                const int k = r0 + i;
+                double2* v = aux + (std::size_t)k * ld + r0;  // v[jj] = reflector entry at column r0 + jj
+                // |row i beyond column i|^2: every thread sums row i over its column
+                // group, so each tile row's total below is the same number
+                double2 part = make_double2(0, 0);
+                for (int jj = i + 1 + ((g - (i + 1) % ng + ng) % ng); jj < n; jj += ng) part.x += cnorm(lds[jj * 16 + i]);
+                const double xn2 = lq_row_sum(part, sh).x;
+                if (tid == 0) {
+                    const double2 alpha = cconj(lds[i * 16 + i]);
+                    if (xn2 == 0.0 && alpha.y == 0.0) {
+                        sh.hh[0] = make_double2(0, 0);
+                        sh.hh[1] = make_double2(0, 0);
+                        sh.beta = alpha.x;
+                    } else {
+                        const double beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+                        sh.hh[0] = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+                        const double2 d = make_double2(alpha.x - beta, alpha.y);  // 1 / (alpha - beta)
+                        const double dd = d.x * d.x + d.y * d.y;
+                        sh.hh[1] = make_double2(d.x / dd, -d.y / dd);
+                        sh.beta = beta;
+                    }
+                }
+                __syncthreads();
+                const double2 tau = sh.hh[0], scale = sh.hh[1];
+                // the reflector goes to aux (for the other tiles) and into tile row i,
+                // whose entries it replaces, for this tile's remaining rows
+                for (int jj = i + tid; jj < n; jj += blockDim.x) {
+                    const double2 e = jj == i ? make_double2(1, 0) : cmul(cconj(lds[jj * 16 + i]), scale);
+                    v[jj] = e;
+                    lds[jj * 16 + i] = e;
+                }
+                if (tid == 0) taus[k] = tau;
+                __syncthreads();
+                lq_apply(lds, n, i, lds + i, 16, tau, i, sh);
+                for (int jj = i + tid; jj < n; jj += blockDim.x)
+                    lds[jj * 16 + i] = jj == i ? make_double2(sh.beta, 0) : make_double2(0, 0);
+                __syncthreads();
+            }
+            lq_store_tile(Wg, ld, lds, q, r0, n, m);
+        }
+        sync();
+        if (live)
+            for (int t = q + 1; t < ntiles; ++t) {
+                if (t % nworkers != me) continue;
+                lq_load_tile(Wg, ld, lds, t, r0, n, m);
+                for (int i = 0; i < nb; ++i) {
+                    const int k = r0 + i;
+                    lq_apply(lds, n, i, aux + (std::size_t)k * ld + r0, 1, taus[k], -1, sh);
+                }
+                lq_store_tile(Wg, ld, lds, t, r0, n, m);
+            }
+    }
+    sync();
+}
+
+
 
 // |x|^2 over the lane-held rows, summed over the wave.
 __device__ inline double col_norm2(const double2 (&x)[kRowsPerLane]) {
@@ -401,6 +535,7 @@ __device__ inline bool pair_rotate_carried(double2 (&x)[kRowsPerLane], double2 (
 // rotated nothing stops working but keeps joining the barriers.
 struct SvdMode {
     int rounds = -1;            // < 0: own tournament length
+    int lq_panels = -1;         // LQ panels (< 0: own ceil(c/16))
     int* gflags = nullptr;      // grid-wide rotation flags, [2]
     bool grid_leader = false;   // clears gflags
     bool active = true;         // false: this group has no matrix
@@ -416,7 +551,8 @@ struct SvdMode {
 template <class Sync>
 __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, int chi, double* __restrict__ sv_out,
                            int* __restrict__ perm_out, double* trunc, int* flags, unsigned long long* stats, int me,
-                           int nworkers, Sync& sync, double2* lds, SvdShared& sh, const SvdMode& mode) {
+                           int nworkers, Sync& sync, double2* lds, SvdShared& sh, const SvdMode& mode,
+                           double2* aux, double2* taus) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int Dl = mode.active ? dims[p] : 1, Dr = mode.active ? dims[p + 2] : 1;
     const int R = 2 * Dl, C = 2 * Dr;
@@ -447,6 +583,10 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
         }
     sync();
     const double floor2 = noise_floor2(__hip_atomic_load(fro, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (aux) {  // W <- L of W = L Q^H (same left singular vectors and values)
+        const int panels = mode.lq_panels >= 0 ? mode.lq_panels : (c + 15) / 16;
+        lq_precondition(Wg, ld, m, c, aux, taus, me, nworkers, sync, lds, sh, panels, mode.active);
+    }
     bool working = mode.active;
     int sweeps_done = 0;
     unsigned long long nvis = 0, nrot = 0;
@@ -611,7 +751,8 @@ struct GridSync {
 __global__ void __launch_bounds__(kDistThreads) k_svd_dist(double2* __restrict__ Wg, int ld, int* dims, int p,
                                                            int chi, double* __restrict__ sv_out,
                                                            int* __restrict__ perm_out, double* trunc, int* flags,
-                                                           unsigned long long* stats, int xcd_stride) {
+                                                           unsigned long long* stats, int xcd_stride, double2* aux,
+                                                           double2* taus) {
     extern __shared__ double2 lds[];
     __shared__ SvdShared sh;
     GridSync sync;
@@ -621,16 +762,19 @@ __global__ void __launch_bounds__(kDistThreads) k_svd_dist(double2* __restrict__
     const int nworkers = (int)(gridDim.x / xcd_stride);
     const int me = worker ? (int)(blockIdx.x / xcd_stride) : nworkers + (int)blockIdx.x;
     const SvdMode mode;
-    svd_blocks(Wg, ld, dims, p, chi, sv_out, perm_out, trunc, flags, stats, me, nworkers, sync, lds, sh, mode);
+    svd_blocks(Wg, ld, dims, p, chi, sv_out, perm_out, trunc, flags, stats, me, nworkers, sync, lds, sh, mode, aux,
+               taus);
 }
 
 // ---- reconstruction (mps.cpp:87-99) ---------------------------------------
 // tall (M = theta):        A'[row,k] = W[:,perm k][row]/S_k;  C'[k,col] = sum_row conj(U[row,k]) M[row,col]
 // wide (M = theta^dagger): C'[k,col] = conj(W[col, perm k]);  A'[row,k] = sum_col conj(M[col,row]) W[col,perm k] / S_k^2
+// With `wts` non-null the right site gets diag(wts) Vh instead of diag(S) Vh
+// (used to build benchmark states with prescribed bond weights).
 __device__ void recon_part(double2* sites, const std::size_t* off, const int* cap, const int* dims, int p,
                            const double2* __restrict__ M, const double2* __restrict__ W, int ld,
                            const double* __restrict__ sv, const int* __restrict__ perm, long long gid,
-                           long long stride) {
+                           long long stride, const double* __restrict__ wts = nullptr) {
     const int Dl = dims[p], keep = dims[p + 1], Dr = dims[p + 2];
     const int R = 2 * Dl, C = 2 * Dr;
     const bool tall = C <= R;
@@ -649,7 +793,9 @@ __device__ void recon_part(double2* sites, const std::size_t* off, const int* ca
             } else {
                 const int k = (int)(idx / C), col = (int)(idx % C);
                 const int s2 = col / Dr, b = col % Dr;
-                Cn[((std::size_t)k * 2 + s2) * sC + b] = cconj(W[(std::size_t)perm[k] * ld + col]);
+                const double2 v = cconj(W[(std::size_t)perm[k] * ld + col]);
+                Cn[((std::size_t)k * 2 + s2) * sC + b] =
+                    wts ? (sv[k] > 0 ? cscale(v, wts[k] / sv[k]) : make_double2(0, 0)) : v;
             }
         } else {
             const long long g = idx - ndirect;
@@ -661,7 +807,8 @@ __device__ void recon_part(double2* sites, const std::size_t* off, const int* ca
                 double2 acc = make_double2(0, 0);
                 for (int row = 0; row < R; ++row) acc = cadd(acc, cmulc(u[row], mc[row]));
                 const int s2 = col / Dr, b = col % Dr;
-                Cn[((std::size_t)k * 2 + s2) * sC + b] = s > 0 ? cscale(acc, 1.0 / s) : make_double2(0, 0);
+                const double f = s > 0 ? (wts ? wts[k] / (s * s) : 1.0 / s) : 0.0;
+                Cn[((std::size_t)k * 2 + s2) * sC + b] = cscale(acc, f);
             } else {
                 const int row = (int)(g / keep), k = (int)(g % keep);
                 const double s = sv[k];
@@ -678,9 +825,10 @@ __device__ void recon_part(double2* sites, const std::size_t* off, const int* ca
 __global__ void __launch_bounds__(256) k_recon(double2* sites, const std::size_t* off, const int* cap,
                                                const int* dims, int p, const double2* __restrict__ M,
                                                const double2* __restrict__ W, int ld,
-                                               const double* __restrict__ sv, const int* __restrict__ perm) {
+                                               const double* __restrict__ sv, const int* __restrict__ perm,
+                                               const double* __restrict__ wts) {
     recon_part(sites, off, cap, dims, p, M, W, ld, sv, perm, (long long)blockIdx.x * blockDim.x + threadIdx.x,
-               (long long)gridDim.x * blockDim.x);
+               (long long)gridDim.x * blockDim.x, wts);
 }
 
 // ---- environments (mps.cpp:150-178, 218-262) ---------------------------------
@@ -983,7 +1131,18 @@ __device__ void theta_tiles(const double2* __restrict__ sites, const std::size_t
 // end of phase 3), with the single-site gates still pending applied.
 struct ExecUpdate {
     int site;   // -1: idle slot
+    int inst;   // instance (MPS) the update belongs to
     Gate4 g;    // two-site gate with pending single-site gates folded in
+};
+// One MPS advanced by the executor (several in a batched run).
+struct ExecInst {
+    double2* sites;
+    const std::size_t* off;
+    const int* cap;
+    int* dims;
+    int chi;
+    double* trunc;               // per-workgroup slots
+    unsigned long long* stats;   // per-workgroup slots
 };
 struct ExecSnap {
     int site;
@@ -991,17 +1150,14 @@ struct ExecSnap {
     double2 g[4];   // 2x2 gate applied to the copy
 };
 struct ExecArgs {
+    const ExecInst* insts;     // instance 0 also owns the snapshots
     const ExecUpdate* ups;     // [nsteps][G]
     const int* rounds;         // [nsteps] tournament rounds of the widest SVD
     const int* snap_begin;     // [nsteps + 1]
     const ExecSnap* snaps;
     int nsteps;
     int G;
-    double2* sites;
-    const std::size_t* off;
-    const int* cap;
-    int* dims;
-    int n, chi;
+    int n;              // sites of instance 0 (snapshots)
     double2* mat;
     double2* work;
     int ld;
@@ -1012,8 +1168,8 @@ struct ExecArgs {
     double2* snap;
     std::size_t snap_stride;
     int* snap_dims;     // [slots][n + 1]
-    double* trunc;      // per-workgroup slots
-    unsigned long long* stats;
+    double2* aux;       // [G][ld * ld] LQ reflectors
+    double2* taus;      // [G][ld]
 };
 
 __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
@@ -1036,16 +1192,18 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
         a.gflags[0] = 0;
         a.gflags[1] = 0;
     }
+    const ExecInst& I0 = a.insts[0];
     for (int st = 0; st < a.nsteps; ++st) {
         const ExecUpdate& u = a.ups[(std::size_t)st * G + grp];
         const bool active = u.site >= 0;
-        // 1. snapshots (whole grid), theta (group)
+        const ExecInst& I = a.insts[active ? u.inst : 0];
+        // 1. snapshots (whole grid, instance 0), theta (group)
         for (int q = a.snap_begin[st]; q < a.snap_begin[st + 1]; ++q) {
             const ExecSnap& sn = a.snaps[q];
             const int i = sn.site;
-            const int Dl = a.dims[i], Dr = a.dims[i + 1], stc = a.cap[i + 1];
-            const double2* src = a.sites + a.off[i];
-            double2* dst = a.snap + (std::size_t)sn.slot * a.snap_stride + a.off[i];
+            const int Dl = I0.dims[i], Dr = I0.dims[i + 1], stc = I0.cap[i + 1];
+            const double2* src = I0.sites + I0.off[i];
+            double2* dst = a.snap + (std::size_t)sn.slot * a.snap_stride + I0.off[i];
             for (long long idx = tid_all; idx < (long long)Dl * Dr; idx += all) {
                 const int x = (int)(idx / Dr), b = (int)(idx % Dr);
                 const double2 v0 = src[((std::size_t)x * 2 + 0) * stc + b];
@@ -1059,18 +1217,19 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 sd[i + 1] = Dr;
             }
         }
-        if (active) theta_tiles(a.sites, a.off, a.cap, a.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
+        if (active) theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
         // 2-3. lockstep SVD; svd_blocks opens with a grid barrier
         SvdMode mode;
         mode.rounds = a.rounds[st];
         mode.gflags = a.gflags;
         mode.grid_leader = blockIdx.x == 0;
         mode.active = active;
-        svd_blocks(W, a.ld, a.dims, active ? u.site : 0, a.chi, sv, perm, a.trunc, sflags, a.stats, rank, nwg, sync,
-                   lds, sh, mode);
+        mode.lq_panels = (a.ld + 15) / 16;
+        svd_blocks(W, a.ld, I.dims, active ? u.site : 0, I.chi, sv, perm, I.trunc, sflags, I.stats, rank, nwg, sync,
+                   lds, sh, mode, a.aux + grp * mat_sz, a.taus + (std::size_t)grp * a.ld);
         sync();
         // 4. reconstruction
-        if (active) recon_part(a.sites, a.off, a.cap, a.dims, u.site, M, W, a.ld, sv, perm, gid, gstride);
+        if (active) recon_part(I.sites, I.off, I.cap, I.dims, u.site, M, W, a.ld, sv, perm, gid, gstride);
         sync();
     }
 }
@@ -1134,6 +1293,8 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     TQ_HIP(hipMemset(stats_.p, 0, 4 * (std::size_t)kStatSlots * sizeof(unsigned long long)));
     mat_.alloc((std::size_t)ld * ld);
     work_.alloc((std::size_t)ld * ld);
+    aux_.alloc((std::size_t)ld * ld);
+    taus_.alloc(ld);
     sv_.alloc(ld);
     perm_.alloc(ld);
     trunc_.alloc(kStatSlots);
@@ -1217,11 +1378,26 @@ void Tebd::randomize(uint64_t seed, double decay) {
                        (unsigned long long)seed, decay);
     TQ_LAUNCH_CHECK();
     TQ_HIP(hipMemcpyAsync(dims_.p, cap_.data(), (n_ + 1) * sizeof(int), hipMemcpyHostToDevice, stream_));
+    // one left-to-right sweep of identity updates: every site but the last
+    // becomes an isometry and bond b carries the weights w_k ~ exp(-k/decay)
+    std::vector<double> w(maxcap_);
+    double tot = 0.0;
+    for (int k = 0; k < maxcap_; ++k) tot += (w[k] = std::exp(-2.0 * k / decay));
+    for (int k = 0; k < maxcap_; ++k) w[k] = std::sqrt(w[k] / tot);
+    DevBuf<double> wd(maxcap_);
+    TQ_HIP(hipMemcpyAsync(wd.p, w.data(), maxcap_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    Gate4 eye;
+    for (auto& x : eye.g) x = make_double2(0, 0);
+    for (int k = 0; k < 4; ++k) eye.g[k * 5] = make_double2(1, 0);
+    const bool timing = timing_;
+    timing_ = false;
+    for (int p = 0; p + 1 < n_; ++p) two_site(p, eye, wd.p);
+    timing_ = timing;
     TQ_HIP(hipMemsetAsync(trunc_.p, 0, kStatSlots * sizeof(double), stream_));
     TQ_HIP(hipStreamSynchronize(stream_));
 }
 
-void Tebd::two_site(int p, const Gate4& g) {
+void Tebd::two_site(int p, const Gate4& g, const double* wts) {
     const int ld = ld_;
     const int Dlc = cap_[p], Drc = cap_[p + 2];
     dim3 tgrid((Drc + 15) / 16, (Dlc + 15) / 16);
@@ -1253,15 +1429,15 @@ void Tebd::two_site(int p, const Gate4& g) {
         int pp = p, cc = chi_;
         unsigned long long* stats = stats_.p;
         int xs = xcd_stride_;
-        void* args[] = {(void*)&work_.p,  (void*)&ld_,     (void*)&dims_.p, (void*)&pp,
-                        (void*)&cc,       (void*)&sv_.p,   (void*)&perm_.p, (void*)&trunc_.p,
-                        (void*)&flags,    (void*)&stats,   (void*)&xs};
+        void* args[] = {(void*)&work_.p, (void*)&ld_,   (void*)&dims_.p,  (void*)&pp,   (void*)&cc,
+                        (void*)&sv_.p,   (void*)&perm_.p, (void*)&trunc_.p, (void*)&flags, (void*)&stats,
+                        (void*)&xs,      (void*)&aux_.p,  (void*)&taus_.p};
         TQ_HIP(hipLaunchCooperativeKernel((const void*)k_svd_dist, dim3(nwg), dim3(kDistThreads), args, shm, stream_));
     }
     if (timing_) TQ_HIP(hipEventRecord(events_[ev_next_++].second, stream_));
     const long long work = (long long)(2 * Dlc) * std::min(chi_, 2 * Drc) + (long long)std::min(chi_, 2 * Dlc) * (2 * Drc);
     hipLaunchKernelGGL(k_recon, dim3((unsigned)grid_for(work, 256)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
-                       caps_dev_.p, dims_.p, p, mat_.p, work_.p, ld, sv_.p, perm_.p);
+                       caps_dev_.p, dims_.p, p, mat_.p, work_.p, ld, sv_.p, perm_.p, wts);
     TQ_LAUNCH_CHECK();
 }
 
@@ -1317,7 +1493,7 @@ void Tebd::step(double A, double B, int rec) {
 void Tebd::run(const double* a, const double* b, int count, int rec0) {
     if (count <= 0) return;
     if (rec0 >= 0) ensure_trace(rec0 + count);
-    if (exec_req_ != 0 && n_ >= 2 && !plan_.empty()) {
+    if (exec_req_ != 0 && n_ >= 2) {
         // snapshots of a recorded chunk stay resident until its entropies are
         // taken; bound the chunk to ~8 GiB of snapshots
         const std::size_t per = off_[n_] * sizeof(double2);
@@ -1337,15 +1513,16 @@ void Tebd::set_executor(int groups) {
     exec_req_ = groups;
 }
 
-// Groups of `per_group` workgroups (enough for the widest block-Jacobi round),
-// as many groups as are co-resident (at most 64), a multiple of 8 when 8 or
+// Groups of `per_group` workgroups (enough for the widest block-Jacobi round of
+// a theta with rows and columns up to 2*maxcap), as many groups as are
+// co-resident (at most `req` if req > 0, else 64), a multiple of 8 when 8 or
 // more fit.
-void Tebd::exec_shape(int& groups, int& per_group, std::size_t& shm) const {
-    const int cmax = 2 * maxcap_;
+static void exec_shape(int maxcap, int req, int& groups, int& per_group, std::size_t& shm) {
+    const int cmax = 2 * maxcap;
     const int nb = (cmax + kBlk - 1) / kBlk;
     const int nbp = std::max(2, nb + (nb & 1));
     per_group = nbp / 2;
-    const std::size_t svd_bytes = (std::size_t)2 * kBlk * (2 * maxcap_) * sizeof(double2);
+    const std::size_t svd_bytes = (std::size_t)2 * kBlk * (2 * maxcap) * sizeof(double2);
     const std::size_t theta_bytes = (std::size_t)(kDistThreads / 256) * 2 * 16 * 2 * 17 * sizeof(double2);
     shm = std::max(svd_bytes, theta_bytes);
     int per_cu = 0, cus = 0, dev = 0;
@@ -1354,7 +1531,7 @@ void Tebd::exec_shape(int& groups, int& per_group, std::size_t& shm) const {
     TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_tebd_lockstep, kDistThreads, shm));
     const long long resident = std::min<long long>((long long)per_cu * cus, kStatSlots);
     long long g = resident / per_group;
-    const long long cap = exec_req_ > 0 ? exec_req_ : 64;
+    const long long cap = req > 0 ? req : 64;
     g = std::min(g, cap);
     if (g >= 8) g -= g % 8;
     if (g < 1) throw Error(-3, "lockstep executor: not enough co-resident workgroups");
@@ -1362,10 +1539,25 @@ void Tebd::exec_shape(int& groups, int& per_group, std::size_t& shm) const {
 }
 
 void Tebd::run_exec(const double* a, const double* b, int count, int rec0) {
-    TQ_HIP(hipStreamSynchronize(stream_));  // the previous run's buffers are free again
+    std::vector<Tebd*> one{this};
+    run_lockstep(one, {a}, {b}, {count}, rec0, exec_req_);
+}
+
+// The lockstep executor over one or several MPS (batched runs): every
+// instance's updates are placed, in its own gate order, into super-steps of at
+// most G mutually independent updates shared by all instances.
+void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const double*>& as,
+                        const std::vector<const double*>& bs, const std::vector<int>& counts, int rec0, int req) {
+    if (ts.empty()) return;
+    if (rec0 >= 0 && ts.size() != 1) throw Error(-1, "entropy recording needs a single instance");
+    Tebd& t0 = *ts[0];
+    for (Tebd* t : ts) TQ_HIP(hipStreamSynchronize(t->stream_));  // previous runs' buffers are free again
+    int maxcap = 1;
+    for (Tebd* t : ts) maxcap = std::max(maxcap, t->maxcap_);
+    const int ld = 2 * maxcap;
     int G = 0, W = 0;
     std::size_t shm = 0;
-    exec_shape(G, W, shm);
+    exec_shape(maxcap, req, G, W, shm);
     using cd = std::complex<double>;
     struct M2 {
         cd u[2][2];
@@ -1379,111 +1571,119 @@ void Tebd::run_exec(const double* a, const double* b, int count, int rec0) {
     };
     auto to_d2 = [](cd z) { return make_double2(z.real(), z.imag()); };
     auto from_d2 = [](double2 z) { return cd(z.x, z.y); };
-    // exp(-i dt/2 g), g = [[-hz, -A], [-A, hz]] (the closed form of k_single_layer)
-    auto half_layer = [&](double A, double B, int i) {
-        const double hz = B * model_.h[i] + (model_.bias.empty() ? 0.0 : A * model_.bias[i]);
-        const double r = std::sqrt(A * A + hz * hz);
-        M2 u = eye;
-        if (r > 0.0) {
-            const double sn = std::sin(0.5 * dt_ * r), cs = std::cos(0.5 * dt_ * r);
-            u.u[0][0] = cd(cs, sn * hz / r);
-            u.u[0][1] = u.u[1][0] = cd(0.0, sn * A / r);
-            u.u[1][1] = cd(cs, -sn * hz / r);
-        }
-        return u;
-    };
-    auto nbp_of = [&](int p) {  // tournament length of the widest matrix update p can see
-        const int c = std::min(2 * cap_[p], 2 * cap_[p + 2]);
-        const int nb = (c + kBlk - 1) / kBlk;
-        return std::max(2, nb + (nb & 1));
-    };
     const Gate4 sw = make_swap_gate();
-    std::vector<M2> pending(n_, eye);
-    std::vector<int> free_at(n_, 0);     // first super-step at which the site may be updated
-    std::vector<int> load;               // updates per super-step
-    std::vector<ExecUpdate> ups;         // [super-step][G]
+    std::vector<int> load;          // updates per super-step
+    std::vector<ExecUpdate> ups;    // [super-step][G]
     std::vector<int> rounds;
     struct SnapAt {
         int at;
         ExecSnap s;
     };
     std::vector<SnapAt> snaps;
-    std::vector<int> last_at(n_, -1);  // super-step of the site's latest update
     int nsteps = 0;
-    auto place = [&](int p, const Gate4& g) {
-        int t = std::max(free_at[p], free_at[p + 1]);
-        while (t < (int)load.size() && load[t] >= G) ++t;
-        if (t >= (int)load.size()) {
-            load.resize(t + 1, 0);
-            rounds.resize(t + 1, 0);
-            ExecUpdate idle{};
-            idle.site = -1;
-            ups.resize((std::size_t)(t + 1) * G, idle);
-        }
-        ExecUpdate& u = ups[(std::size_t)t * G + load[t]++];
-        u.site = p;
-        u.g = g;
-        rounds[t] = std::max(rounds[t], nbp_of(p) - 1);
-        free_at[p] = free_at[p + 1] = t + 1;
-        nsteps = std::max(nsteps, t + 1);
-        return t;
-    };
-    for (int k = 0; k < count; ++k) {
-        const double A = a[k], B = b[k];
-        std::vector<M2> u1(n_);
-        for (int i = 0; i < n_; ++i) {
-            u1[i] = half_layer(A, B, i);
-            pending[i] = mul2(u1[i], pending[i]);
-        }
-        const double sub = dt_ / ns_;
-        auto add_edge = [&](const std::pair<std::size_t, std::size_t>& span) {
-            for (std::size_t q = span.first; q < span.second; ++q) {
-                const Update& u = plan_[q];
-                const Gate4 g = u.edge < 0 ? sw : make_zz_gate(-B * model_.ew[u.edge] * sub);
-                const M2& P = pending[u.site];
-                const M2& Q = pending[u.site + 1];
-                Gate4 f;
-                for (int r = 0; r < 4; ++r)
-                    for (int c = 0; c < 4; ++c) {
-                        cd acc = 0.0;
-                        for (int s = 0; s < 4; ++s)
-                            acc += from_d2(g.g[r * 4 + s]) * P.u[s >> 1][c >> 1] * Q.u[s & 1][c & 1];
-                        f.g[r * 4 + c] = to_d2(acc);
-                    }
-                pending[u.site] = eye;
-                pending[u.site + 1] = eye;
-                const int t = place(u.site, f);
-                last_at[u.site] = last_at[u.site + 1] = t;
+    ExecUpdate idle{};
+    idle.site = -1;
+    idle.inst = 0;
+    std::vector<std::vector<M2>> pendings(ts.size());
+    for (std::size_t in = 0; in < ts.size(); ++in) {
+        Tebd& T = *ts[in];
+        const int n = T.n_;
+        const double* a = as[in];
+        const double* b = bs[in];
+        // exp(-i dt/2 g), g = [[-hz, -A], [-A, hz]] (the closed form of k_single_layer)
+        auto half_layer = [&](double A, double B, int i) {
+            const double hz = B * T.model_.h[i] + (T.model_.bias.empty() ? 0.0 : A * T.model_.bias[i]);
+            const double r = std::sqrt(A * A + hz * hz);
+            M2 u = eye;
+            if (r > 0.0) {
+                const double sn = std::sin(0.5 * T.dt_ * r), cs = std::cos(0.5 * T.dt_ * r);
+                u.u[0][0] = cd(cs, sn * hz / r);
+                u.u[0][1] = u.u[1][0] = cd(0.0, sn * A / r);
+                u.u[1][1] = cd(cs, -sn * hz / r);
             }
+            return u;
         };
-        for (int s = 0; s < ns_; ++s) {
-            if (s % 2 == 0)
-                for (const auto& span : spans_) add_edge(span);
-            else
-                for (std::size_t e = spans_.size(); e-- > 0;) add_edge(spans_[e]);
-        }
-        for (int i = 0; i < n_; ++i) pending[i] = mul2(u1[i], pending[i]);
-        if (rec0 >= 0)
-            for (int i = 0; i < n_; ++i) {
-                // after the site's last update of this step (sites change only in
-                // the reconstruction phase), before any later update changes it
-                SnapAt x;
-                x.at = last_at[i] + 1;
-                x.s.site = i;
-                x.s.slot = k;
-                x.s.g[0] = to_d2(pending[i].u[0][0]);
-                x.s.g[1] = to_d2(pending[i].u[0][1]);
-                x.s.g[2] = to_d2(pending[i].u[1][0]);
-                x.s.g[3] = to_d2(pending[i].u[1][1]);
-                snaps.push_back(x);
+        auto nbp_of = [&](int p) {  // tournament length of the widest matrix update p can see
+            const int c = std::min(2 * T.cap_[p], 2 * T.cap_[p + 2]);
+            const int nb = (c + kBlk - 1) / kBlk;
+            return std::max(2, nb + (nb & 1));
+        };
+        std::vector<M2>& pending = pendings[in];
+        pending.assign(n, eye);
+        std::vector<int> free_at(n, 0);   // first super-step at which the site may be updated
+        std::vector<int> last_at(n, -1);  // super-step of the site's latest update
+        auto place = [&](int p, const Gate4& g) {
+            int t = std::max(free_at[p], free_at[p + 1]);
+            while (t < (int)load.size() && load[t] >= G) ++t;
+            if (t >= (int)load.size()) {
+                load.resize(t + 1, 0);
+                rounds.resize(t + 1, 0);
+                ups.resize((std::size_t)(t + 1) * G, idle);
             }
+            ExecUpdate& u = ups[(std::size_t)t * G + load[t]++];
+            u.site = p;
+            u.inst = (int)in;
+            u.g = g;
+            rounds[t] = std::max(rounds[t], nbp_of(p) - 1);
+            free_at[p] = free_at[p + 1] = t + 1;
+            nsteps = std::max(nsteps, t + 1);
+            return t;
+        };
+        for (int k = 0; k < counts[in]; ++k) {
+            const double A = a[k], B = b[k];
+            std::vector<M2> u1(n);
+            for (int i = 0; i < n; ++i) {
+                u1[i] = half_layer(A, B, i);
+                pending[i] = mul2(u1[i], pending[i]);
+            }
+            const double sub = T.dt_ / T.ns_;
+            auto add_edge = [&](const std::pair<std::size_t, std::size_t>& span) {
+                for (std::size_t q = span.first; q < span.second; ++q) {
+                    const Update& u = T.plan_[q];
+                    const Gate4 g = u.edge < 0 ? sw : make_zz_gate(-B * T.model_.ew[u.edge] * sub);
+                    const M2& P = pending[u.site];
+                    const M2& Q = pending[u.site + 1];
+                    Gate4 f;
+                    for (int r = 0; r < 4; ++r)
+                        for (int c = 0; c < 4; ++c) {
+                            cd acc = 0.0;
+                            for (int s = 0; s < 4; ++s)
+                                acc += from_d2(g.g[r * 4 + s]) * P.u[s >> 1][c >> 1] * Q.u[s & 1][c & 1];
+                            f.g[r * 4 + c] = to_d2(acc);
+                        }
+                    pending[u.site] = eye;
+                    pending[u.site + 1] = eye;
+                    const int t = place(u.site, f);
+                    last_at[u.site] = last_at[u.site + 1] = t;
+                }
+            };
+            for (int s = 0; s < T.ns_; ++s) {
+                if (s % 2 == 0)
+                    for (const auto& span : T.spans_) add_edge(span);
+                else
+                    for (std::size_t e = T.spans_.size(); e-- > 0;) add_edge(T.spans_[e]);
+            }
+            for (int i = 0; i < n; ++i) pending[i] = mul2(u1[i], pending[i]);
+            if (rec0 >= 0)
+                for (int i = 0; i < n; ++i) {
+                    // after the site's last update of this step (sites change only in
+                    // the reconstruction phase), before any later update changes it
+                    SnapAt x;
+                    x.at = last_at[i] + 1;
+                    x.s.site = i;
+                    x.s.slot = k;
+                    x.s.g[0] = to_d2(pending[i].u[0][0]);
+                    x.s.g[1] = to_d2(pending[i].u[0][1]);
+                    x.s.g[2] = to_d2(pending[i].u[1][0]);
+                    x.s.g[3] = to_d2(pending[i].u[1][1]);
+                    snaps.push_back(x);
+                }
+        }
     }
     // snapshots due after the last super-step get a trailing super-step of their own
     int total_steps = nsteps;
     for (const auto& x : snaps) total_steps = std::max(total_steps, x.at + 1);
-    if (total_steps > nsteps) {
-        ExecUpdate idle{};
-        idle.site = -1;
+    if (total_steps > (int)rounds.size()) {
         ups.resize((std::size_t)total_steps * G, idle);
         rounds.resize(total_steps, 0);
     }
@@ -1498,90 +1698,109 @@ void Tebd::run_exec(const double* a, const double* b, int count, int rec0) {
     // a super-step whose slots are all idle still runs one (empty) round
     for (int t = 0; t < total_steps; ++t) rounds[t] = std::max(rounds[t], 1);
 
-    // device buffers (grown on demand, kept across runs)
-    const std::size_t mat_sz = (std::size_t)ld_ * ld_;
-    if (ex_mat_.n < G * mat_sz) {
-        ex_mat_.alloc(G * mat_sz);
-        ex_work_.alloc(G * mat_sz);
-        ex_sv_.alloc((std::size_t)G * ld_);
-        ex_perm_.alloc((std::size_t)G * ld_);
-        ex_flags_.alloc(4 * (std::size_t)G + 2);
+    // device buffers of instance 0 (grown on demand, kept across runs)
+    const std::size_t mat_sz = (std::size_t)ld * ld;
+    if (t0.ex_mat_.n < G * mat_sz) {
+        t0.ex_mat_.alloc(G * mat_sz);
+        t0.ex_work_.alloc(G * mat_sz);
+        t0.ex_aux_.alloc(G * mat_sz);
     }
-    if (rec0 >= 0 && ex_snap_.n < (std::size_t)count * off_[n_]) {
-        ex_snap_.alloc((std::size_t)count * off_[n_]);
-        ex_sdims_.alloc((std::size_t)count * (n_ + 1));
+    if (t0.ex_sv_.n < (std::size_t)G * ld) {
+        t0.ex_sv_.alloc((std::size_t)G * ld);
+        t0.ex_perm_.alloc((std::size_t)G * ld);
+        t0.ex_taus_.alloc((std::size_t)G * ld);
+        t0.ex_flags_.alloc(4 * (std::size_t)G + 2);
     }
-    const std::size_t up_bytes = ups.size() * sizeof(ExecUpdate);
-    const std::size_t sn_bytes = std::max<std::size_t>(1, snap_list.size()) * sizeof(ExecSnap);
+    const int count0 = counts[0];
+    if (rec0 >= 0 && t0.ex_snap_.n < (std::size_t)count0 * t0.off_[t0.n_]) {
+        t0.ex_snap_.alloc((std::size_t)count0 * t0.off_[t0.n_]);
+        t0.ex_sdims_.alloc((std::size_t)count0 * (t0.n_ + 1));
+    }
+    std::vector<ExecInst> insts(ts.size());
+    for (std::size_t in = 0; in < ts.size(); ++in) {
+        Tebd& T = *ts[in];
+        insts[in] = ExecInst{T.sites_.p, T.off_dev_.p, T.caps_dev_.p, T.dims_.p, T.chi_, T.trunc_.p, T.stats_.p};
+    }
+    auto align = [](std::size_t x) { return (x + 255) & ~(std::size_t)255; };
+    const std::size_t in_bytes = align(insts.size() * sizeof(ExecInst));
+    const std::size_t up_bytes = align(ups.size() * sizeof(ExecUpdate));
+    const std::size_t sn_bytes = align(std::max<std::size_t>(1, snap_list.size()) * sizeof(ExecSnap));
     const std::size_t int_count = (std::size_t)total_steps + (total_steps + 1);
-    ex_host_.resize(up_bytes + sn_bytes + int_count * sizeof(int));
-    unsigned char* hb = ex_host_.data();
-    std::memcpy(hb, ups.data(), up_bytes);
-    if (!snap_list.empty()) std::memcpy(hb + up_bytes, snap_list.data(), snap_list.size() * sizeof(ExecSnap));
-    int* hi = reinterpret_cast<int*>(hb + up_bytes + sn_bytes);
+    t0.ex_host_.assign(in_bytes + up_bytes + sn_bytes + int_count * sizeof(int), 0);
+    unsigned char* hb = t0.ex_host_.data();
+    std::memcpy(hb, insts.data(), insts.size() * sizeof(ExecInst));
+    std::memcpy(hb + in_bytes, ups.data(), ups.size() * sizeof(ExecUpdate));
+    if (!snap_list.empty())
+        std::memcpy(hb + in_bytes + up_bytes, snap_list.data(), snap_list.size() * sizeof(ExecSnap));
+    int* hi = reinterpret_cast<int*>(hb + in_bytes + up_bytes + sn_bytes);
     std::memcpy(hi, rounds.data(), total_steps * sizeof(int));
     std::memcpy(hi + total_steps, snap_begin.data(), (total_steps + 1) * sizeof(int));
-    if (ex_tasks_.n < ex_host_.size()) ex_tasks_.alloc(ex_host_.size());
-    TQ_HIP(hipMemcpy(ex_tasks_.p, hb, ex_host_.size(), hipMemcpyHostToDevice));
+    if (t0.ex_tasks_.n < t0.ex_host_.size()) t0.ex_tasks_.alloc(t0.ex_host_.size());
+    TQ_HIP(hipMemcpy(t0.ex_tasks_.p, hb, t0.ex_host_.size(), hipMemcpyHostToDevice));
 
     ExecArgs x{};
-    x.ups = reinterpret_cast<const ExecUpdate*>(ex_tasks_.p);
-    x.snaps = reinterpret_cast<const ExecSnap*>(ex_tasks_.p + up_bytes);
-    x.rounds = reinterpret_cast<const int*>(ex_tasks_.p + up_bytes + sn_bytes);
+    x.insts = reinterpret_cast<const ExecInst*>(t0.ex_tasks_.p);
+    x.ups = reinterpret_cast<const ExecUpdate*>(t0.ex_tasks_.p + in_bytes);
+    x.snaps = reinterpret_cast<const ExecSnap*>(t0.ex_tasks_.p + in_bytes + up_bytes);
+    x.rounds = reinterpret_cast<const int*>(t0.ex_tasks_.p + in_bytes + up_bytes + sn_bytes);
     x.snap_begin = x.rounds + total_steps;
     x.nsteps = total_steps;
     x.G = G;
-    x.sites = sites_.p;
-    x.off = off_dev_.p;
-    x.cap = caps_dev_.p;
-    x.dims = dims_.p;
-    x.n = n_;
-    x.chi = chi_;
-    x.mat = ex_mat_.p;
-    x.work = ex_work_.p;
-    x.ld = ld_;
-    x.sv = ex_sv_.p;
-    x.perm = ex_perm_.p;
-    x.svd_flags = ex_flags_.p;
-    x.gflags = ex_flags_.p + 4 * G;
-    x.snap = ex_snap_.p;
-    x.snap_stride = off_[n_];
-    x.snap_dims = ex_sdims_.p;
-    x.trunc = trunc_.p;
-    x.stats = stats_.p;
+    x.n = t0.n_;
+    x.mat = t0.ex_mat_.p;
+    x.work = t0.ex_work_.p;
+    x.ld = ld;
+    x.sv = t0.ex_sv_.p;
+    x.perm = t0.ex_perm_.p;
+    x.svd_flags = t0.ex_flags_.p;
+    x.gflags = t0.ex_flags_.p + 4 * G;
+    x.snap = t0.ex_snap_.p;
+    x.snap_stride = t0.off_[t0.n_];
+    x.snap_dims = t0.ex_sdims_.p;
+    x.aux = t0.ex_aux_.p;
+    x.taus = t0.ex_taus_.p;
     void* args[] = {(void*)&x};
-    if (timing_) {
-        if (ev_next_ >= events_.size()) {
+    hipStream_t st = t0.stream_;
+    if (t0.timing_) {
+        if (t0.ev_next_ >= t0.events_.size()) {
             hipEvent_t e0, e1;
             TQ_HIP(hipEventCreate(&e0));
             TQ_HIP(hipEventCreate(&e1));
-            events_.push_back({e0, e1});
+            t0.events_.push_back({e0, e1});
         }
-        TQ_HIP(hipEventRecord(events_[ev_next_].first, stream_));
+        TQ_HIP(hipEventRecord(t0.events_[t0.ev_next_].first, st));
     }
-    TQ_HIP(hipLaunchCooperativeKernel((const void*)k_tebd_lockstep, dim3((unsigned)(G * W)), dim3(kDistThreads), args,
-                                      shm, stream_));
-    if (timing_) TQ_HIP(hipEventRecord(events_[ev_next_++].second, stream_));
-    exec_supersteps_ += total_steps;
-    for (int t = 0; t < nsteps; ++t) exec_updates_ += load[t];
+    if (total_steps > 0)
+        TQ_HIP(hipLaunchCooperativeKernel((const void*)k_tebd_lockstep, dim3((unsigned)(G * W)), dim3(kDistThreads),
+                                          args, shm, st));
+    if (t0.timing_) TQ_HIP(hipEventRecord(t0.events_[t0.ev_next_++].second, st));
+    t0.exec_supersteps_ += total_steps;
+    for (int t = 0; t < (int)load.size(); ++t) t0.exec_updates_ += load[t];
     // fold the single-site gates still pending into the sites
-    std::vector<double2> g(4 * (std::size_t)n_);
-    for (int i = 0; i < n_; ++i) {
-        g[4 * i + 0] = to_d2(pending[i].u[0][0]);
-        g[4 * i + 1] = to_d2(pending[i].u[0][1]);
-        g[4 * i + 2] = to_d2(pending[i].u[1][0]);
-        g[4 * i + 3] = to_d2(pending[i].u[1][1]);
+    std::vector<std::vector<double2>> gates(ts.size());
+    for (std::size_t in = 0; in < ts.size(); ++in) {
+        Tebd& T = *ts[in];
+        std::vector<double2>& g = gates[in];
+        g.resize(4 * (std::size_t)T.n_);
+        for (int i = 0; i < T.n_; ++i) {
+            const M2& P = pendings[in][i];
+            g[4 * i + 0] = to_d2(P.u[0][0]);
+            g[4 * i + 1] = to_d2(P.u[0][1]);
+            g[4 * i + 2] = to_d2(P.u[1][0]);
+            g[4 * i + 3] = to_d2(P.u[1][1]);
+        }
+        if (!T.ex_gates_.p) T.ex_gates_.alloc(4 * (std::size_t)T.n_);
+        TQ_HIP(hipMemcpyAsync(T.ex_gates_.p, g.data(), g.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_apply_site_gates, dim3(T.n_), dim3(256), 0, st, T.sites_.p, T.off_dev_.p, T.caps_dev_.p,
+                           T.dims_.p, T.ex_gates_.p);
+        TQ_LAUNCH_CHECK();
     }
-    if (!ex_gates_.p) ex_gates_.alloc(4 * (std::size_t)n_);
-    TQ_HIP(hipMemcpyAsync(ex_gates_.p, g.data(), g.size() * sizeof(double2), hipMemcpyHostToDevice, stream_));
-    hipLaunchKernelGGL(k_apply_site_gates, dim3(n_), dim3(256), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p,
-                       dims_.p, ex_gates_.p);
-    TQ_LAUNCH_CHECK();
     // entropies of the recorded steps from their snapshots
     if (rec0 >= 0)
-        for (int k = 0; k < count; ++k)
-            entropy_of(ex_snap_.p + (std::size_t)k * off_[n_], ex_sdims_.p + (std::size_t)k * (n_ + 1), rec0 + k);
-    TQ_HIP(hipStreamSynchronize(stream_));  // host staging buffers go out of scope
+        for (int k = 0; k < count0; ++k)
+            t0.entropy_of(t0.ex_snap_.p + (std::size_t)k * t0.off_[t0.n_],
+                          t0.ex_sdims_.p + (std::size_t)k * (t0.n_ + 1), rec0 + k);
+    TQ_HIP(hipStreamSynchronize(st));  // host staging buffers go out of scope
 }
 
 void Tebd::env_right(const double2* rin, int j, double2* rout, const double2* sites, const int* dims) {
@@ -1758,6 +1977,35 @@ Tebd::Readout Tebd::readout() {
     TQ_HIP(hipMemcpyAsync(&amp, res.p + n_ + m + 1, sizeof(double2), hipMemcpyDeviceToHost, stream_));
     sync();
     out.prob = nrm > 0 ? (amp.x * amp.x + amp.y * amp.y) / nrm : 0.0;
+    return out;
+}
+
+// theta of sites (p, p+1) under the identity gate, as the SVD sees it: R x C
+// = 2Dl x 2Dr row-major complex (interleaved re/im), R and C in rows/cols.
+std::vector<double> Tebd::theta(int p, int& rows, int& cols) {
+    if (p < 0 || p + 1 >= n_) throw Error(-1, "theta site out of range");
+    Gate4 eye;
+    for (auto& x : eye.g) x = make_double2(0, 0);
+    for (int k = 0; k < 4; ++k) eye.g[k * 5] = make_double2(1, 0);
+    const std::vector<int> d = bonds();
+    const int Dl = d[p], Dr = d[p + 2];
+    rows = 2 * Dl;
+    cols = 2 * Dr;
+    dim3 tgrid((cap_[p + 2] + 15) / 16, (cap_[p] + 15) / 16);
+    hipLaunchKernelGGL(k_theta, tgrid, dim3(16, 16), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p, p, eye,
+                       mat_.p, work_.p, ld_);
+    TQ_LAUNCH_CHECK();
+    std::vector<double2> m((std::size_t)ld_ * ld_);
+    TQ_HIP(hipMemcpyAsync(m.data(), mat_.p, m.size() * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    sync();
+    const bool tall = cols <= rows;  // M = theta column-major, else theta^dagger column-major
+    std::vector<double> out(2 * (std::size_t)rows * cols);
+    for (int r = 0; r < rows; ++r)
+        for (int c = 0; c < cols; ++c) {
+            const double2 v = tall ? m[(std::size_t)c * ld_ + r] : m[(std::size_t)r * ld_ + c];
+            out[2 * ((std::size_t)r * cols + c)] = v.x;
+            out[2 * ((std::size_t)r * cols + c) + 1] = tall ? v.y : -v.y;
+        }
     return out;
 }
 

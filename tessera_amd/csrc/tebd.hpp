@@ -43,7 +43,9 @@ public:
     Tebd& operator=(const Tebd&) = delete;
 
     void reset_product();                       // |+>^n
-    void randomize(uint64_t seed, double decay);  // saturated random state
+    // Random MPS with every bond at capacity, then one left-to-right sweep of
+    // identity updates giving bond weights ~exp(-k/decay) (a saturated state).
+    void randomize(uint64_t seed, double decay);
     // Enqueue one Trotter step (mps.cpp:309-356); entropy recorded at slot `rec` if >= 0.
     void step(double A, double B, int rec);
     // Enqueue `count` steps with schedule values a[k], b[k]; when rec0 >= 0 the
@@ -54,6 +56,10 @@ public:
     // -1 = as many groups as are co-resident (capped at 64).
     void set_executor(int groups);
     int executor_groups() const { return exec_req_; }
+    // Batched lockstep run: instance i advances counts[i] steps with schedule
+    // values as[i][k], bs[i][k]; entropy recording (rec0 >= 0) needs one instance.
+    static void run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const double*>& as,
+                             const std::vector<const double*>& bs, const std::vector<int>& counts, int rec0, int req);
     void sync();
     std::vector<double> entropy(int count);      // recorded slots [0, count)
     double max_trunc();
@@ -66,6 +72,7 @@ public:
     };
     Readout readout();                           // mps.cpp:358-380
     std::vector<double> dense();                 // n <= 24, interleaved re/im
+    std::vector<double> theta(int p, int& rows, int& cols);  // identity-gate theta at (p, p+1)
     std::size_t updates_per_step() const { return plan_.size() * (std::size_t)ns_; }
     int n() const { return n_; }
     hipStream_t stream() const { return stream_; }
@@ -76,8 +83,7 @@ public:
 private:
     void build_plan();
     void run_exec(const double* a, const double* b, int count, int rec0);
-    void exec_shape(int& groups, int& per_group, std::size_t& shm) const;
-    void two_site(int p, const Gate4& g);
+    void two_site(int p, const Gate4& g, const double* wts = nullptr);
     void entropy_sweep(int rec);
     void entropy_of(const double2* sites, const int* dims, int rec);
     void env_left(const double2* lin, int site, bool z, double2* lout);
@@ -101,6 +107,7 @@ private:
     DevBuf<int> caps_dev_;
     DevBuf<std::size_t> off_dev_;
     DevBuf<double2> mat_, work_;   // theta (kept) and the Jacobi working copy
+    DevBuf<double2> aux_, taus_;   // LQ reflectors of the distributed SVD
     DevBuf<double> sv_;            // sorted singular values
     DevBuf<int> perm_;             // sorted position -> column
     DevBuf<double> trunc_;         // running max discarded weight
@@ -124,7 +131,7 @@ private:
     int exec_req_ = 0;
     long long exec_supersteps_ = 0, exec_updates_ = 0;
     DevBuf<unsigned char> ex_tasks_;   // schedule: updates, snapshots, rounds, snapshot offsets
-    DevBuf<double2> ex_mat_, ex_work_, ex_snap_, ex_gates_;
+    DevBuf<double2> ex_mat_, ex_work_, ex_snap_, ex_gates_, ex_aux_, ex_taus_;
     DevBuf<double> ex_sv_;
     DevBuf<int> ex_perm_, ex_flags_, ex_sdims_;
     std::vector<unsigned char> ex_host_;

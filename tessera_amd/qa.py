@@ -79,6 +79,10 @@ def lib() -> ctypes.CDLL:
         L.tessera_tebd_set_timing.argtypes = [vp, ctypes.c_int]
         L.tessera_tebd_set_executor.restype = ctypes.c_int
         L.tessera_tebd_set_executor.argtypes = [vp, ctypes.c_int]
+        L.tessera_tebd_theta.restype = ctypes.c_int
+        L.tessera_tebd_theta.argtypes = [vp, _usz, vp, _usz, vp, vp]
+        L.tessera_tebd_batch_run.restype = ctypes.c_int
+        L.tessera_tebd_batch_run.argtypes = [vp, _usz, vp, vp, vp, ctypes.c_int]
         L.tessera_tebd_exec_stats.restype = ctypes.c_int
         L.tessera_tebd_exec_stats.argtypes = [vp, vp, vp]
         L.tessera_tebd_updates_per_step.restype = _usz
@@ -307,6 +311,17 @@ class TebdSession:
         per run() with at most g concurrent groups of workgroups; -1: as many as fit."""
         _check(lib().tessera_tebd_set_executor(self._h, int(groups)))
 
+    def theta(self, p: int) -> np.ndarray:
+        """Identity-gate theta of sites (p, p+1): the (2Dl, 2Dr) matrix the next
+        update at p factorises."""
+        r = ctypes.c_int32()
+        c = ctypes.c_int32()
+        _check(lib().tessera_tebd_theta(self._h, p, None, 0, ctypes.addressof(r), ctypes.addressof(c)))
+        out = np.zeros(2 * r.value * c.value)
+        _check(lib().tessera_tebd_theta(self._h, p, _p(out), r.value * c.value, ctypes.addressof(r),
+                                        ctypes.addressof(c)))
+        return (out[0::2] + 1j * out[1::2]).reshape(r.value, c.value)
+
     def exec_stats(self) -> dict:
         """Lockstep executor totals: super-steps run and updates placed in them."""
         ss = ctypes.c_int64()
@@ -317,3 +332,26 @@ class TebdSession:
     @property
     def updates_per_step(self) -> int:
         return int(lib().tessera_tebd_updates_per_step(self._h))
+
+
+def batch_run(sessions, schedules, groups: int = -1):
+    """Advance several TebdSession objects together (tessera_tebd_batch_run):
+    session i by the steps of schedules[i] (an object with .a and .b arrays)."""
+    k = len(sessions)
+    if k == 0:
+        return
+    keep = []
+    a_ptrs = (ctypes.c_void_p * k)()
+    b_ptrs = (ctypes.c_void_p * k)()
+    steps = np.zeros(k, np.uintp)
+    handles = (ctypes.c_void_p * k)()
+    for i, (s, sch) in enumerate(zip(sessions, schedules)):
+        a = np.ascontiguousarray(sch.a, np.float64)
+        b = np.ascontiguousarray(sch.b, np.float64)
+        keep += [a, b]
+        a_ptrs[i] = a.ctypes.data
+        b_ptrs[i] = b.ctypes.data
+        steps[i] = a.shape[0]
+        handles[i] = s._h
+    _check(lib().tessera_tebd_batch_run(ctypes.addressof(handles), k, ctypes.addressof(a_ptrs),
+                                        ctypes.addressof(b_ptrs), _p(steps), int(groups)))

@@ -51,8 +51,10 @@ def parse():
     ap.add_argument("--chi", type=int, default=CHI)
     ap.add_argument("--executor", type=int, default=0,
                     help="lockstep executor groups (0: one launch sequence per update, -1: as many as fit)")
-    ap.add_argument("--workload", choices=("tebd", "sv"), default="tebd",
-                    help="tebd: the headline metric; sv: the n=30 state-vector sweep (HBM roofline)")
+    ap.add_argument("--workload", choices=("tebd", "sv", "batch"), default="tebd",
+                    help="tebd: the headline metric; sv: the n=30 state-vector sweep (HBM roofline); "
+                         "batch: batched anneals of config-4 instances")
+    ap.add_argument("--instances", type=int, default=32, help="batch workload: number of config-4 instances")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (PMC byte-counter calibration) and exit")
     return ap.parse_args()
@@ -137,6 +139,74 @@ def sv_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_bench(args):
+    """Config-4 instances (n ~ U{16..96}, chi in {32, 64, 128}, banded sparse
+    graphs), each from a saturated state (bonds at capacity, bond weights
+    exp(-k/40)), advanced by `steps` mid-anneal Trotter steps: all together in
+    one batched launch vs one at a time on the per-update path."""
+    import numpy as np
+
+    from tessera_amd import instances as I
+    from tessera_amd import qa
+
+    insts = I.config4(args.instances)
+
+    def sessions():
+        out = []
+        for k, x in enumerate(insts):
+            s = qa.TebdSession(x.model, x.chi, x.dt)
+            s.randomize(4000 + k, 40.0)
+            out.append(s)
+        return out
+
+    def sched(x):
+        k0 = x.sched.steps // 2
+        return I.Schedule(x.sched.a[k0:k0 + args.steps], x.sched.b[k0:k0 + args.steps])
+
+    solo_times = []
+    ss = sessions()
+    for s, x in zip(ss, insts):
+        sc = sched(x)
+        t0 = time.perf_counter()
+        s.run(sc.a, sc.b, record=False)
+        s.sync()
+        solo_times.append(time.perf_counter() - t0)
+    total_updates = sum(s.updates_per_step * args.steps for s in ss)
+    for s in ss:
+        s.close()
+    t_solo = sum(solo_times)
+    progress(f"one at a time: {len(insts)} instances x {args.steps} steps in {t_solo:.2f}s")
+    ss = sessions()
+    t0 = time.perf_counter()
+    qa.batch_run(ss, [sched(x) for x in insts], args.executor or -1)
+    for s in ss:
+        s.sync()
+    t_batch = time.perf_counter() - t0
+    ex = ss[0].exec_stats()
+    for s in ss:
+        s.close()
+    progress(f"batched: {t_batch:.2f}s")
+    st = np.array(solo_times)
+    out = {
+        "metric": "batched TEBD instances/s (config-4 sample, saturated, mid-anneal steps)",
+        "value": len(insts) / t_batch, "unit": "instances/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": t_batch / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 (fp64)", "data": "synthetic: config-4 generator (seeds 4000+i), saturated states",
+        "config": {"workload": f"{len(insts)} instances x {args.steps} Trotter steps, chi in {{32,64,128}}",
+                   "two_site_updates": total_updates,
+                   "batched_updates_per_s": total_updates / t_batch,
+                   "one_at_a_time_updates_per_s": total_updates / t_solo,
+                   "one_at_a_time_instances_per_s": len(insts) / t_solo,
+                   "speedup": t_solo / t_batch,
+                   "one_at_a_time_instance_s_p50": float(np.percentile(st, 50)),
+                   "one_at_a_time_instance_s_p99": float(np.percentile(st, 99)),
+                   "batched_instance_s": t_batch,
+                   "supersteps": ex["supersteps"],
+                   "updates_per_superstep": ex["updates"] / max(ex["supersteps"], 1)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -148,6 +218,9 @@ def main():
         return
     if args.workload == "sv":
         sv_bench(args)
+        return
+    if args.workload == "batch":
+        batch_bench(args)
         return
     import numpy as np
 

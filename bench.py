@@ -11,13 +11,18 @@ Workload (one "step" = one symmetric Trotter step of the MPS-TEBD anneal):
   left-isometries with bond weights ~exp(-k/40), so every interior update
   factorises a 512 x 512 complex128 theta and keeps 256 of its singular values.
 
+By default the timed steps run in the lockstep executor (one cooperative launch
+in which consecutive Trotter steps overlap; --executor 0 runs one launch
+sequence per update instead).
+
 Output: one JSON line. `value` is two-site updates per second over the timed
-steps (inputs resident on the device). `roofline` covers the dominant kernel, the
-one-sided Jacobi SVD: algorithmic FP64 flops = 16 per visited column-pair row +
-18 per rotated one (counted on the device), divided by the HIP-event time of the
-SVD launches. `cpu_baseline` times the bit-exact CPU port of the upstream engine
-on one host core for a bounded sample (one Jacobi sweep of a saturated update);
-see DESIGN.md for the sample and the sweep count it is scaled by.
+steps (inputs resident on the device). `roofline` covers the dominant kernel
+(the executor launch, or the SVD launches on the per-update path): algorithmic
+FP64 flops of its one-sided Jacobi = 16 per visited column-pair row + 18 per
+rotated one (counted on the device), divided by the kernel's HIP-event time.
+`cpu_baseline` times the bit-exact CPU port of the upstream engine on one host
+core for a bounded sample (one Jacobi sweep of a saturated update), scaled by
+the 25 sweeps the upstream eigensolver needs on it.
 """
 from __future__ import annotations
 
@@ -44,13 +49,14 @@ REF_SWEEPS_CHI256 = 25
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="untimed steps (the saturated-state preparation already runs every kernel)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--n", type=int, default=N_SPINS)
     ap.add_argument("--chi", type=int, default=CHI)
-    ap.add_argument("--executor", type=int, default=0,
-                    help="lockstep executor groups (0: one launch sequence per update, -1: as many as fit)")
+    ap.add_argument("--executor", type=int, default=-1,
+                    help="lockstep executor groups (-1: as many as fit, 0: one launch sequence per update)")
     ap.add_argument("--workload", choices=("tebd", "sv", "batch"), default="tebd",
                     help="tebd: the headline metric; sv: the n=30 state-vector sweep (HBM roofline); "
                          "batch: batched anneals of config-4 instances")

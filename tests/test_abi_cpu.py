@@ -2,6 +2,7 @@
 include/tessera_qa.h declares, and rejects bad arguments before any device work
 (upstream semantics for tessera_solve_qa: 0.0 on null pointers, NaN for n > 24,
 rust/src/capi.rs:121-126)."""
+import ctypes
 import math
 import re
 import subprocess
@@ -96,3 +97,23 @@ def test_quantum_anneal_rejects_degenerate_schedule():
     with pytest.raises(qa.QaFailure) as ei:
         qa.quantum_anneal(I.make_ising([1.0], []), I.Schedule(np.array([1.0]), np.array([0.0])), 32)
     assert ei.value.kind is qa.QaError.BadSchedule
+
+
+def test_session_entries_reject_null_sessions():
+    """Session, executor, batch and diagnostic entries validate their arguments
+    before touching a device."""
+    L = qa.lib()
+    assert L.tessera_tebd_set_executor(None, 0) == -1
+    assert L.tessera_tebd_exec_stats(None, None, None) == -1
+    r = np.zeros(1, np.int32)
+    assert L.tessera_tebd_theta(None, 0, None, 0, qa._p(r), qa._p(r)) == -1
+    assert L.tessera_tebd_batch_run(None, 1, None, None, None, -1) == -1
+    handles = (ctypes.c_void_p * 1)(None)
+    steps = np.zeros(1, np.uintp)
+    ptrs = (ctypes.c_void_p * 1)(None)
+    rc = L.tessera_tebd_batch_run(ctypes.addressof(handles), 1, ctypes.addressof(ptrs), ctypes.addressof(ptrs),
+                                  qa._p(steps), -1)
+    assert rc == -1 and b"null session" in L.tessera_last_error()
+    assert L.tessera_tebd_batch_run(ctypes.addressof(handles), 0, None, None, None, -2) == -1
+    assert L.tessera_tebd_batch_run(ctypes.addressof(handles), 0, None, None, None, -1) == 0  # nothing to do
+    assert qa.lib().tessera_tebd_updates_per_step(None) == 0

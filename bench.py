@@ -57,10 +57,16 @@ def parse():
     ap.add_argument("--chi", type=int, default=CHI)
     ap.add_argument("--executor", type=int, default=-1,
                     help="lockstep executor groups (-1: as many as fit, 0: one launch sequence per update)")
-    ap.add_argument("--workload", choices=("tebd", "sv", "batch"), default="tebd",
+    ap.add_argument("--workload", choices=("tebd", "sv", "batch", "learned"), default="tebd",
                     help="tebd: the headline metric; sv: the n=30 state-vector sweep (HBM roofline); "
-                         "batch: batched anneals of config-4 instances")
+                         "batch: batched anneals of config-4 instances; learned: guidance network "
+                         "(GPU TEBD targets, GPU training, guided vs plain anneals)")
+    ap.add_argument("--train", type=int, default=512, help="learned workload: training instances")
+    ap.add_argument("--eval", type=int, default=64, help="learned workload: held-out instances")
+    ap.add_argument("--epochs", type=int, default=40, help="learned workload: training epochs")
     ap.add_argument("--instances", type=int, default=32, help="batch workload: number of config-4 instances")
+    ap.add_argument("--exec-mode", type=int, default=0, choices=(0, 1, 2),
+                    help="batch workloads: executor layout 0 auto, 1 grouped, 2 solo (one workgroup per update)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (PMC byte-counter calibration) and exit")
     return ap.parse_args()
@@ -184,7 +190,7 @@ def batch_bench(args):
     progress(f"one at a time: {len(insts)} instances x {args.steps} steps in {t_solo:.2f}s")
     ss = sessions()
     t0 = time.perf_counter()
-    qa.batch_run(ss, [sched(x) for x in insts], args.executor or -1)
+    qa.batch_run(ss, [sched(x) for x in insts], args.executor or -1, args.exec_mode)
     for s in ss:
         s.sync()
     t_batch = time.perf_counter() - t0
@@ -207,8 +213,91 @@ def batch_bench(args):
                    "one_at_a_time_instance_s_p50": float(np.percentile(st, 50)),
                    "one_at_a_time_instance_s_p99": float(np.percentile(st, 99)),
                    "batched_instance_s": t_batch,
-                   "supersteps": ex["supersteps"],
+                   "exec_mode": args.exec_mode, "supersteps": ex["supersteps"],
                    "updates_per_superstep": ex["updates"] / max(ex["supersteps"], 1)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def learned_bench(args):
+    """Learned guidance, a reduced config-5 sample: training instances generated
+    like the batched workload (n ~ U{16..40}, chi = 32, T = 10 at dt = 0.05),
+    targets from batched GPU TEBD anneals (final <Z_i>, entanglement-peak
+    fraction), the network trained on the GPU, then held-out instances (n ~
+    U{32..64}) annealed plain at full length and guided at shorter lengths.
+    A guided run succeeds when its readout energy is within 1% of the best plain
+    energy; the wall time includes the network inference."""
+    import numpy as np
+
+    from tessera_amd import learned as Lrn
+
+    steps, dt, chi = 200, 0.05, 32
+    train = Lrn.guidance_instances(args.train, 5_000_000, 16, 40, chi, steps, dt)
+    evals = Lrn.guidance_instances(args.eval, 6_000_000, 32, 64, chi, steps, dt)
+    progress(f"targets for {len(train)} training instances")
+    tg = Lrn.tebd_targets(train, chunks=10, log=progress)
+    progress(f"targets: {tg.seconds:.1f}s")
+    batch = Lrn.pack([x.model for x in train], tg.z, tg.s_peak)
+    net = Lrn.GuidanceNet(seed=5)
+    net.set_data(batch)
+    l0, _ = net.grad()
+    net.fit(1, batch_graphs=64, lr=2e-3, seed=0)  # warm-up epoch (code objects, buffers)
+    net.params = Lrn.GuidanceNet(seed=5).params
+    t0 = time.perf_counter()
+    hist = net.fit(args.epochs, batch_graphs=64, lr=2e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    progress(f"trained {args.epochs} epochs in {t_train:.1f}s: loss {l0:.4f} -> {hist[-1]:.4f}")
+    base = np.array([x.sched.steps for x in evals])
+    e_plain, _, w_plain = Lrn.anneal_batch(evals, base, log=progress)
+    progress(f"plain full-length anneals: {w_plain:.1f}s")
+    rows, plain_e = [], [e_plain]
+    for frac in (0.125, 0.25, 0.5):
+        g = Lrn.guidance(net, [x.model for x in evals], base, frac, 1.0, float(tg.s_peak.mean()))
+        e_g, _, w_g = Lrn.anneal_batch(evals, g.steps, g.bias, log=progress)
+        short = np.maximum(2, np.round(frac * base).astype(int))
+        e_p, _, w_p = Lrn.anneal_batch(evals, short, log=progress)
+        plain_e.append(e_p)
+        rows.append(dict(fraction=frac, guided_steps=int(g.steps.sum()), guided_wall_s=w_g + g.infer_seconds,
+                         infer_s=g.infer_seconds, plain_short_wall_s=w_p, e_guided=e_g, e_plain_short=e_p))
+        progress(f"fraction {frac}: guided {w_g:.1f}s, plain short {w_p:.1f}s")
+    best = np.min(np.stack(plain_e), axis=0)
+    table = []
+    pick = None
+    for r in rows:
+        ok_g = Lrn.within(r.pop("e_guided"), best)
+        ok_p = Lrn.within(r.pop("e_plain_short"), best)
+        r.update(guided_within_1pct=float(ok_g.mean()), plain_short_within_1pct=float(ok_p.mean()),
+                 wall_ratio=r["guided_wall_s"] / w_plain)
+        table.append(r)
+        if pick is None and r["guided_within_1pct"] >= float(Lrn.within(e_plain, best).mean()):
+            pick = r
+    # CPU baseline: the FP64 oracle's forward + backward of one 64-graph batch on one core
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import gnn_ref  # the CPU oracle: only the baseline leg uses it
+
+    sub = Lrn.sub_batch(batch, 0, 64)
+    w = net.params.astype(np.float64)
+    tc = time.perf_counter()
+    gnn_ref.loss_and_grad(w, sub)
+    t_cpu = time.perf_counter() - tc
+    n_graphs = args.epochs * len(train)
+    out = {
+        "metric": "time-to-1% energy, learned vs plain (guided wall / plain wall; reduced config-5 sample)",
+        "value": pick["wall_ratio"] if pick else None, "unit": "ratio", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 MFMA / fp32 accumulate (network); c128 (TEBD)",
+        "data": "synthetic: banded sparse Ising graphs (seeds 5000000+i train, 6000000+i eval)",
+        "config": {"workload": f"{len(train)} training instances n 16..40, {len(evals)} held-out n 32..64, chi={chi}, "
+                               f"T={steps * dt:g} at dt={dt}",
+                   "target_seconds": tg.seconds, "train_seconds": t_train,
+                   "train_graphs_per_s": n_graphs / t_train,
+                   "train_nodes_per_s": args.epochs * int(batch["node_off"][-1]) / t_train,
+                   "loss_initial": l0, "loss_final": hist[-1],
+                   "plain_full_wall_s": w_plain, "plain_full_within_1pct": float(Lrn.within(e_plain, best).mean()),
+                   "fractions": table},
+        "roofline": None,
+        "cpu_baseline": {"value": 64.0 / t_cpu, "unit": "training graphs/s (forward + backward)", "cores": 1,
+                         "kind": "port", "sample": "FP64 numpy oracle, one 64-graph batch"},
     }
     print(json.dumps(out), flush=True)
 
@@ -227,6 +316,9 @@ def main():
         return
     if args.workload == "batch":
         batch_bench(args)
+        return
+    if args.workload == "learned":
+        learned_bench(args)
         return
     import numpy as np
 

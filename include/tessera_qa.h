@@ -116,6 +116,10 @@ int tessera_tebd_create(const double* h, size_t n, const size_t* ei, const size_
                         tessera_tebd** out);
 /* |+>^n product state (mps.cpp:9-17). */
 int tessera_tebd_reset(tessera_tebd* t);
+/* Warm start: the product of the single-spin ground states of
+ * -(X_i + bias_i Z_i), the ground state at s = 0 of an anneal run with the
+ * same bias (tessera_tebd_set_bias); bias = NULL gives |+>^n. */
+int tessera_tebd_reset_warm(tessera_tebd* t, const double* bias);
 /* Benchmark state: random site tensors with every bond at its capacity
  * min(chi, 2^b, 2^(n-b)), then one left-to-right sweep of identity two-site
  * updates that leaves every site but the last an isometry and gives bond b the
@@ -131,6 +135,15 @@ int tessera_tebd_sync(tessera_tebd* t, double* out_entropy, size_t cap, double* 
                       int32_t* out_bonds);
 /* Readout: spins, energy, expected energy, prob (as tessera_qa_anneal_ex). */
 int tessera_tebd_readout(tessera_tebd* t, int8_t* out_s, double* out_scalars);
+/* Per-spin expectations <Z_i> = <psi|Z_i|psi> / <psi|psi> of the current state
+ * (n values; the quantities whose signs the readout takes, mps.cpp:369-372). */
+int tessera_tebd_expect_z(tessera_tebd* t, double* out_z);
+/* Qubit-0 entanglement entropy of the current state (mps.cpp:218-291). */
+int tessera_tebd_entropy_now(tessera_tebd* t, double* out_entropy);
+/* Per-spin longitudinal warm-start field for the steps enqueued from now on:
+ * the single-site generator becomes -A X - (B h_i + A bias_i) Z (n values);
+ * NULL removes it (the plain anneal). */
+int tessera_tebd_set_bias(tessera_tebd* t, const double* bias);
 /* Dense amplitudes (n <= 24): out[2*x], out[2*x+1] = re, im of psi(x), where bit
  * i of x is the physical index of site i (0 <-> spin +1). */
 int tessera_tebd_dense(tessera_tebd* t, double* out);
@@ -166,12 +179,59 @@ int tessera_tebd_set_executor(tessera_tebd* t, int groups);
  * entropy is recorded; read each session out with tessera_tebd_readout. */
 int tessera_tebd_batch_run(tessera_tebd* const* sessions, size_t count, const double* const* a,
                            const double* const* b, const size_t* steps, int groups);
+/* tessera_tebd_batch_run with the executor layout chosen explicitly: mode 0 =
+ * automatic (solo for four or more sessions with chi <= 32, grouped otherwise), 1 =
+ * grouped (each update's SVD spread over a group of workgroups that advance in
+ * lockstep: few, large updates), 2 = solo (one workgroup per update, one grid
+ * barrier per super-step, up to one update per resident workgroup, thetas up
+ * to 64 x 64 factorised in LDS: many independent small updates, as in batches
+ * of small instances). groups caps the
+ * concurrent updates (-1: as many as fit). Results agree to rounding. */
+int tessera_tebd_batch_run_ex(tessera_tebd* const* sessions, size_t count, const double* const* a,
+                              const double* const* b, const size_t* steps, int groups, int mode);
 /* Lockstep executor totals since creation: super-steps run and two-site
  * updates placed in them (updates / (super-steps * groups) = slot fill). */
 int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* out_updates);
 /* Number of two-site updates one Trotter step performs for this model. */
 size_t tessera_tebd_updates_per_step(const tessera_tebd* t);
 void tessera_tebd_destroy(tessera_tebd* t);
+
+/* ---- learned guidance network (no upstream counterpart; the structure of
+ * python/gnn_guide.py:81-103 IsingGNN is the template) ----
+ * A 3-layer message-passing network of width 64 over Ising graphs, BF16 on
+ * MFMA with FP32 accumulation and FP32 master weights (Adam). Node features
+ * (h_i, deg_i / 3), edge feature J_ij. Outputs: per-spin z_i in (-1, 1), the
+ * estimate of the final <Z_i> of the anneal (the warm-start bias), and per
+ * graph s_g in (0, 1), the estimate of the schedule fraction at which the
+ * qubit-0 entanglement entropy peaks (the schedule stretch is derived from it).
+ * Loss: mean_i (z_i - tz_i)^2 + mean_g (s_g - ts_g)^2. */
+typedef struct tessera_gnn tessera_gnn;
+
+/* Fresh network: uniform(+-1/sqrt(fan_in)) weights from `seed`, zero biases. */
+int tessera_gnn_create(uint64_t seed, tessera_gnn** out);
+/* Number of FP32 parameters (the flat layout of get/set_params). */
+size_t tessera_gnn_param_count(void);
+int tessera_gnn_get_params(tessera_gnn* g, float* out);
+/* Replace the master weights (and their BF16 copies); resets the Adam state. */
+int tessera_gnn_set_params(tessera_gnn* g, const float* in);
+/* Device-resident dataset of `graphs` graphs. Graph k owns nodes
+ * node_off[k]..node_off[k+1]-1 (at least one); node_feat holds 2 floats per
+ * node; node i's neighbours are adj_nbr[adj_off[i]..adj_off[i+1]-1] (global
+ * node ids inside the same graph; list both directions of every edge) with
+ * couplings adj_j. z_target (per node) and s_target (per graph) may both be
+ * NULL for inference only. */
+int tessera_gnn_set_data(tessera_gnn* g, size_t graphs, const int32_t* node_off, const float* node_feat,
+                         const int32_t* adj_off, const int32_t* adj_nbr, const float* adj_j,
+                         const float* z_target, const float* s_target);
+/* Forward pass over graphs [g0, g1): out_z per node of the range, out_s per graph
+ * (either may be NULL). */
+int tessera_gnn_predict(tessera_gnn* g, size_t g0, size_t g1, float* out_z, float* out_s);
+/* Loss of graphs [g0, g1) and its gradient (param_count floats, nullable); no update. */
+int tessera_gnn_grad(tessera_gnn* g, size_t g0, size_t g1, float* out_grad, double* out_loss);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on the loss of graphs [g0, g1);
+ * out_loss (nullable) receives the loss before the step. */
+int tessera_gnn_train_step(tessera_gnn* g, size_t g0, size_t g1, float lr, double* out_loss);
+void tessera_gnn_destroy(tessera_gnn* g);
 
 /* Text of the last error on this thread ("" if none). */
 const char* tessera_last_error(void);

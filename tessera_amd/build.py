@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 LIBDIR = HERE / "_lib"
 LIB = LIBDIR / "libtessera_qa.so"
 
-SOURCES = ["sv.hip", "tebd.hip", "abi.cpp"]
+SOURCES = ["sv.hip", "tebd.hip", "gnn.hip", "abi.cpp"]
 ARCH = os.environ.get("TESSERA_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     f"--offload-arch={ARCH}",

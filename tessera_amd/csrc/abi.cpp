@@ -1,12 +1,14 @@
 // extern "C" entry points of libtessera_qa.so (declared in include/tessera_qa.h).
 // Every entry point catches all exceptions: nothing unwinds across the C ABI.
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <string>
 
 #include "../../include/tessera_qa.h"
 #include "common.hpp"
+#include "gnn.hpp"
 #include "sv.hpp"
 #include "tebd.hpp"
 
@@ -63,6 +65,10 @@ int status_of(const std::exception& e) {
 struct tessera_tebd {
     std::unique_ptr<tq::Tebd> eng;
     int recorded = 0;
+};
+
+struct tessera_gnn {
+    std::unique_ptr<tq::Gnn> net;
 };
 
 using namespace tq;
@@ -289,6 +295,54 @@ int tessera_tebd_readout(tessera_tebd* t, int8_t* out_s, double* out_scalars) {
     }
 }
 
+int tessera_tebd_reset_warm(tessera_tebd* t, const double* bias) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        if (bias && !finite_all(bias, (std::size_t)t->eng->n())) throw Error(TESSERA_ERR_NONFINITE, "non-finite bias");
+        t->eng->reset_product(bias);
+        t->recorded = 0;
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_expect_z(tessera_tebd* t, double* out_z) {
+    try {
+        if (!t || !out_z) throw Error(TESSERA_ERR_ARG, "null argument");
+        const Tebd::Readout r = t->eng->readout();
+        std::memcpy(out_z, r.z.data(), r.z.size() * sizeof(double));
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_entropy_now(tessera_tebd* t, double* out_entropy) {
+    try {
+        if (!t || !out_entropy) throw Error(TESSERA_ERR_ARG, "null argument");
+        *out_entropy = t->eng->entropy_now();
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_tebd_set_bias(tessera_tebd* t, const double* bias) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        if (bias && !finite_all(bias, (std::size_t)t->eng->n())) throw Error(TESSERA_ERR_NONFINITE, "non-finite bias");
+        t->eng->set_bias(bias);
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
 int tessera_tebd_dense(tessera_tebd* t, double* out) {
     try {
         if (!t || !out) throw Error(TESSERA_ERR_ARG, "null argument");
@@ -366,7 +420,13 @@ int tessera_tebd_set_executor(tessera_tebd* t, int groups) {
 
 int tessera_tebd_batch_run(tessera_tebd* const* sessions, size_t count, const double* const* a,
                            const double* const* b, const size_t* steps, int groups) {
+    return tessera_tebd_batch_run_ex(sessions, count, a, b, steps, groups, 0);
+}
+
+int tessera_tebd_batch_run_ex(tessera_tebd* const* sessions, size_t count, const double* const* a,
+                              const double* const* b, const size_t* steps, int groups, int mode) {
     try {
+        if (mode < 0 || mode > 2) throw Error(TESSERA_ERR_ARG, "mode must be 0, 1 or 2");
         if (!sessions || (count && (!a || !b || !steps))) throw Error(TESSERA_ERR_ARG, "null argument");
         if (groups < -1) throw Error(TESSERA_ERR_ARG, "groups must be >= -1");
         std::vector<Tebd*> ts;
@@ -388,7 +448,7 @@ int tessera_tebd_batch_run(tessera_tebd* const* sessions, size_t count, const do
             bs.push_back(b[i]);
             cs.push_back((int)steps[i]);
         }
-        Tebd::run_lockstep(ts, as, bs, cs, -1, groups);
+        Tebd::run_lockstep(ts, as, bs, cs, -1, groups, mode);
         return TESSERA_OK;
     } catch (const std::exception& e) {
         set_last_error(e.what());
@@ -408,6 +468,102 @@ int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* o
         return status_of(e);
     }
 }
+
+int tessera_gnn_create(uint64_t seed, tessera_gnn** out) {
+    try {
+        if (!out) throw Error(TESSERA_ERR_ARG, "null argument");
+        *out = nullptr;
+        auto g = std::make_unique<tessera_gnn>();
+        g->net = std::make_unique<Gnn>(seed);
+        *out = g.release();
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+size_t tessera_gnn_param_count(void) { return (size_t)GnnLayout::COUNT; }
+
+int tessera_gnn_get_params(tessera_gnn* g, float* out) {
+    try {
+        if (!g || !out) throw Error(TESSERA_ERR_ARG, "null argument");
+        g->net->get_params(out);
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_gnn_set_params(tessera_gnn* g, const float* in) {
+    try {
+        if (!g || !in) throw Error(TESSERA_ERR_ARG, "null argument");
+        g->net->set_params(in);
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_gnn_set_data(tessera_gnn* g, size_t graphs, const int32_t* node_off, const float* node_feat,
+                         const int32_t* adj_off, const int32_t* adj_nbr, const float* adj_j,
+                         const float* z_target, const float* s_target) {
+    try {
+        if (!g || !node_off || !node_feat || !adj_off) throw Error(TESSERA_ERR_ARG, "null argument");
+        if (graphs == 0 || graphs > (size_t)INT32_MAX) throw Error(TESSERA_ERR_ARG, "graphs out of range");
+        if ((z_target == nullptr) != (s_target == nullptr))
+            throw Error(TESSERA_ERR_ARG, "z_target and s_target must both be set or both be NULL");
+        if (node_off[graphs] <= 0) throw Error(TESSERA_ERR_ARG, "no nodes");
+        if (adj_off[node_off[graphs]] > 0 && (!adj_nbr || !adj_j)) throw Error(TESSERA_ERR_ARG, "null adjacency");
+        g->net->set_data((int)graphs, node_off, node_feat, adj_off, adj_nbr, adj_j, z_target, s_target);
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_gnn_predict(tessera_gnn* g, size_t g0, size_t g1, float* out_z, float* out_s) {
+    try {
+        if (!g) throw Error(TESSERA_ERR_ARG, "null network");
+        if (g0 >= g1 || g1 > (size_t)g->net->graphs()) throw Error(TESSERA_ERR_ARG, "bad graph range");
+        g->net->predict((int)g0, (int)g1, out_z, out_s);
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_gnn_grad(tessera_gnn* g, size_t g0, size_t g1, float* out_grad, double* out_loss) {
+    try {
+        if (!g) throw Error(TESSERA_ERR_ARG, "null network");
+        if (g0 >= g1 || g1 > (size_t)g->net->graphs()) throw Error(TESSERA_ERR_ARG, "bad graph range");
+        const double loss = g->net->grad((int)g0, (int)g1, out_grad);
+        if (out_loss) *out_loss = loss;
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+int tessera_gnn_train_step(tessera_gnn* g, size_t g0, size_t g1, float lr, double* out_loss) {
+    try {
+        if (!g) throw Error(TESSERA_ERR_ARG, "null network");
+        if (g0 >= g1 || g1 > (size_t)g->net->graphs()) throw Error(TESSERA_ERR_ARG, "bad graph range");
+        const double loss = g->net->train_step((int)g0, (int)g1, lr);
+        if (out_loss) *out_loss = loss;
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
+void tessera_gnn_destroy(tessera_gnn* g) { delete g; }
 
 size_t tessera_tebd_updates_per_step(const tessera_tebd* t) { return t ? t->eng->updates_per_step() : 0; }
 

@@ -542,8 +542,8 @@ struct SvdMode {
 };
 
 // Block-Jacobi SVD run by `nworkers` workgroups that synchronise with `sync()`.
-// Worker `me` < P takes block pair `me` of each tournament round; workers with
-// me >= P only join the barriers. The leader (me == 0) sorts, truncates and
+// Worker `me` takes block pairs me, me + nworkers, ... of each tournament round
+// (P = nbp / 2 pairs); workers with me >= P only join the barriers. The leader (me == 0) sorts, truncates and
 // writes dims[p+1], sv_out, perm_out; callers must synchronise again before
 // reading them. Accounting goes to this workgroup's slots: stats[4*b + 0..3] =
 // calls, sweeps, visited and rotated pair-rows, trunc[b] = max discarded weight
@@ -593,9 +593,12 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     for (int sweep = 0; sweep < 40; ++sweep) {
         if (working) ++sweeps_done;
         for (int r = 0; r < rounds; ++r) {
-            if (working && r < nbp - 1 && me < P) {
+            // worker `me` takes pairs me, me + nworkers, ... of the round (one
+            // pair when there are P workers, all of them for a lone workgroup)
+            if (working && r < nbp - 1)
+            for (int pi = me; pi < P; pi += nworkers) {
                 int bi, bj;
-                rr_pair(r, me, nbp, bi, bj);
+                rr_pair(r, pi, nbp, bi, bj);
                 if (tid == 0) sh.rot = 0;
                 // LDS column q <-> block (q < 8 ? bi : bj), member q & 7
                 for (int q = wave; q < 2 * kBlk; q += kDistThreads / 64) {
@@ -675,6 +678,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                         for (int row = lane; row < m; row += 64) dst[row] = src[row];
                     }
                 }
+                __syncthreads();  // the next pair of this worker reuses the LDS columns
             }
             sync();
         }
@@ -737,6 +741,154 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
         dims[p + 1] = keep;
         slot[0] += 1;
         slot[1] += (unsigned long long)sweeps_done;
+    }
+    for (int q = tid; q < c; q += blockDim.x) {
+        sv_out[q] = sh.nrm[sh.perm[q]];
+        perm_out[q] = sh.perm[q];
+    }
+}
+
+// ---- small SVD held in LDS (one workgroup per update) ----------------------
+// For theta with c <= kSmall columns of m <= kSmall rows (bond dimensions up to
+// 32, or larger bonds not yet grown): the whole working copy sits in LDS and
+// every 16-lane group rotates one column pair per tournament round, so a round
+// costs one workgroup barrier instead of the block pipeline of svd_blocks.
+// Rotation, convergence test, noise floor, ordering and keep rule are those of
+// pair_rotate and svd_blocks (mps.cpp:77-85 for the keep rule).
+constexpr int kSmall = 64;
+constexpr int kSmallRows = kSmall / 16;  // rows per lane of a 16-lane group
+
+__device__ inline double sum16(double v) {  // over the aligned 16-lane group
+    v += __shfl_xor(v, 8, 64);
+    v += __shfl_xor(v, 4, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 1, 64);
+    return v;
+}
+
+__device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, int chi, int m, int c,
+                          double* __restrict__ sv_out, int* __restrict__ perm_out, double* trunc,
+                          unsigned long long* stats, double2* A, SvdShared& sh) {
+    const int tid = threadIdx.x, l16 = tid & 15, grp = tid >> 4, ngrp = blockDim.x >> 4;
+    for (int idx = tid; idx < m * c; idx += blockDim.x) {
+        const int col = idx / m, row = idx % m;
+        A[idx] = Wg[(std::size_t)col * ld + row];
+    }
+    if (tid == 0) {
+        sh.vis = 0;
+        sh.rots = 0;
+    }
+    __syncthreads();
+    double fro = 0.0;
+    for (int idx = tid; idx < m * c; idx += blockDim.x) fro += cnorm(A[idx]);
+    fro = wave_sum(fro);
+    if ((tid & 63) == 0) sh.red[tid >> 6][0].x = fro;
+    __syncthreads();
+    fro = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) fro += sh.red[w][0].x;
+    const double floor2 = noise_floor2(fro), tol = jacobi_tol(m);
+    const int cp = c + (c & 1);
+    int sweeps = 0;
+    unsigned long long nvis = 0, nrot = 0;
+    for (int sweep = 0; sweep < 60 && cp >= 2; ++sweep) {
+        ++sweeps;
+        if (tid == 0) sh.rot = 0;
+        __syncthreads();
+        for (int r = 0; r < cp - 1; ++r) {
+            for (int k = grp; k < cp / 2; k += ngrp) {
+                int i, j;
+                rr_pair(r, k, cp, i, j);
+                if (i >= c || j >= c) continue;
+                double2* px = A + (std::size_t)i * m;
+                double2* py = A + (std::size_t)j * m;
+                double2 x[kSmallRows], y[kSmallRows];
+                double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+#pragma unroll
+                for (int q = 0; q < kSmallRows; ++q) {
+                    const int row = l16 + 16 * q;
+                    x[q] = row < m ? px[row] : make_double2(0, 0);
+                    y[q] = row < m ? py[row] : make_double2(0, 0);
+                    al += cnorm(x[q]);
+                    be += cnorm(y[q]);
+                    gr += x[q].x * y[q].x + x[q].y * y[q].y;  // conj(x) y
+                    gi += x[q].x * y[q].y - x[q].y * y[q].x;
+                }
+                al = sum16(al);
+                be = sum16(be);
+                gr = sum16(gr);
+                gi = sum16(gi);
+                ++nvis;
+                if (al <= floor2 || be <= floor2) continue;
+                const double gabs = sqrt(gr * gr + gi * gi);
+                if (!(gabs > 0.0) || gabs <= tol * sqrt(al * be)) continue;
+                const double2 ph = make_double2(gr / gabs, -gi / gabs);
+                const double zeta = (be - al) / (2.0 * gabs);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+#pragma unroll
+                for (int q = 0; q < kSmallRows; ++q) {
+                    const int row = l16 + 16 * q;
+                    const double2 u = x[q], v = cmul(y[q], ph);
+                    if (row < m) {
+                        px[row] = make_double2(cs * u.x - sn * v.x, cs * u.y - sn * v.y);
+                        py[row] = make_double2(sn * u.x + cs * v.x, sn * u.y + cs * v.y);
+                    }
+                }
+                ++nrot;
+                if (l16 == 0) sh.rot = 1;
+            }
+            __syncthreads();
+        }
+        const int f = sh.rot;
+        __syncthreads();
+        if (!f) break;
+    }
+    for (int col = grp; col < c; col += ngrp) {
+        double s = 0.0;
+        for (int row = l16; row < m; row += 16) s += cnorm(A[(std::size_t)col * m + row]);
+        s = sum16(s);
+        if (l16 == 0) sh.nrm[col] = sqrt(s);
+    }
+    if (l16 == 0 && nvis) {
+        atomicAdd(&sh.vis, nvis * (unsigned long long)m);  // LDS
+        atomicAdd(&sh.rots, nrot * (unsigned long long)m);
+    }
+    // the rotated columns go back to the working copy the reconstruction reads
+    for (int idx = tid; idx < m * c; idx += blockDim.x) {
+        const int col = idx / m, row = idx % m;
+        Wg[(std::size_t)col * ld + row] = A[idx];
+    }
+    __syncthreads();
+    for (int q = tid; q < c; q += blockDim.x) {  // stable descending rank
+        const double v = sh.nrm[q];
+        int rank = 0;
+        for (int o = 0; o < c; ++o) {
+            const double u = sh.nrm[o];
+            rank += (u > v) || (u == v && o < q);
+        }
+        sh.perm[rank] = q;
+    }
+    __syncthreads();
+    unsigned long long* slot = stats + 4 * (std::size_t)blockIdx.x;
+    if (tid == 0) {
+        double total = 0.0;
+        for (int q = 0; q < c; ++q) {
+            const double s = sh.nrm[sh.perm[q]];
+            total += s * s;
+        }
+        int keep = min(chi, c);
+        while (keep > 1 && sh.nrm[sh.perm[keep - 1]] < 1e-14) --keep;
+        double disc = 0.0;
+        for (int q = keep; q < c; ++q) {
+            const double s = sh.nrm[sh.perm[q]];
+            disc += s * s;
+        }
+        if (total > 0) trunc[blockIdx.x] = fmax(trunc[blockIdx.x], disc / total);
+        dims[p + 1] = keep;
+        slot[0] += 1;
+        slot[1] += (unsigned long long)sweeps;
+        slot[2] += sh.vis;
+        slot[3] += sh.rots;
     }
     for (int q = tid; q < c; q += blockDim.x) {
         sv_out[q] = sh.nrm[sh.perm[q]];
@@ -1151,7 +1303,8 @@ struct ExecSnap {
 };
 struct ExecArgs {
     const ExecInst* insts;     // instance 0 also owns the snapshots
-    const ExecUpdate* ups;     // [nsteps][G]
+    const ExecUpdate* ups;     // updates of super-step st: ups[up_begin[st] .. up_begin[st+1])
+    const int* up_begin;       // [nsteps + 1]
     const int* rounds;         // [nsteps] tournament rounds of the widest SVD
     const int* snap_begin;     // [nsteps + 1]
     const ExecSnap* snaps;
@@ -1170,6 +1323,17 @@ struct ExecArgs {
     int* snap_dims;     // [slots][n + 1]
     double2* aux;       // [G][ld * ld] LQ reflectors
     double2* taus;      // [G][ld]
+    int solo;           // 1: one workgroup per group, workgroup barriers inside an update
+    int small;          // solo: theta up to kSmall x kSmall takes svd_small (dynamic LDS fits it)
+};
+
+// Barrier of one workgroup for code written against a barrier functor: the
+// fence makes global-memory writes of every wave visible to the others.
+struct BlockSync {
+    __device__ void operator()() const {
+        __threadfence();
+        __syncthreads();
+    }
 };
 
 __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
@@ -1194,8 +1358,8 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
     }
     const ExecInst& I0 = a.insts[0];
     for (int st = 0; st < a.nsteps; ++st) {
-        const ExecUpdate& u = a.ups[(std::size_t)st * G + grp];
-        const bool active = u.site >= 0;
+        const bool active = grp < a.up_begin[st + 1] - a.up_begin[st];
+        const ExecUpdate& u = a.ups[active ? a.up_begin[st] + grp : 0];
         const ExecInst& I = a.insts[active ? u.inst : 0];
         // 1. snapshots (whole grid, instance 0), theta (group)
         for (int q = a.snap_begin[st]; q < a.snap_begin[st + 1]; ++q) {
@@ -1216,6 +1380,28 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 sd[i] = Dl;
                 sd[i + 1] = Dr;
             }
+        }
+        if (a.solo) {
+            // the snapshots read sites that this super-step's reconstructions may rewrite
+            if (a.snap_begin[st + 1] > a.snap_begin[st]) sync();
+            if (active) {
+                BlockSync bs;
+                theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, 0, 1, lds);
+                bs();
+                const int R = 2 * I.dims[u.site], C = 2 * I.dims[u.site + 2];
+                const int m = C <= R ? R : C, c = C <= R ? C : R;
+                if (a.small && m <= kSmall && c <= kSmall) {
+                    svd_small(W, a.ld, I.dims, u.site, I.chi, m, c, sv, perm, I.trunc, I.stats, lds, sh);
+                } else {
+                    SvdMode mode;  // own tournament length, own convergence
+                    svd_blocks(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, 0, 1, bs, lds,
+                               sh, mode, a.aux + grp * mat_sz, a.taus + (std::size_t)grp * a.ld);
+                }
+                bs();
+                recon_part(I.sites, I.off, I.cap, I.dims, u.site, M, W, a.ld, sv, perm, tid, blockDim.x);
+            }
+            sync();  // the super-step's sites are final
+            continue;
         }
         if (active) theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
         // 2-3. lockstep SVD; svd_blocks opens with a grid barrier
@@ -1356,12 +1542,21 @@ void Tebd::ensure_trace(int count) {
     }
 }
 
-void Tebd::reset_product() {
+void Tebd::reset_product(const double* warm) {
     std::vector<double2> host(off_[n_], make_double2(0, 0));
     const double amp = 1.0 / std::sqrt(2.0);
     for (int i = 0; i < n_; ++i) {
-        host[off_[i] + 0 * cap_[i + 1]] = make_double2(amp, 0);  // (0,0,0)
-        host[off_[i] + 1 * cap_[i + 1]] = make_double2(amp, 0);  // (0,1,0)
+        double a0 = amp, a1 = amp;
+        if (warm) {
+            // ground state of -(X + b Z): (1, r - b) / norm with r = sqrt(1 + b^2)
+            const double b = warm[i], r = std::sqrt(1.0 + b * b);
+            const double c1 = b >= 0.0 ? 1.0 / (r + b) : r - b;
+            const double nrm = std::sqrt(1.0 + c1 * c1);
+            a0 = 1.0 / nrm;
+            a1 = c1 / nrm;
+        }
+        host[off_[i] + 0 * cap_[i + 1]] = make_double2(a0, 0);  // (0,0,0)
+        host[off_[i] + 1 * cap_[i + 1]] = make_double2(a1, 0);  // (0,1,0)
     }
     TQ_HIP(hipMemcpyAsync(sites_.p, host.data(), host.size() * sizeof(double2), hipMemcpyHostToDevice, stream_));
     std::vector<int> ones(n_ + 1, 1);
@@ -1517,21 +1712,23 @@ void Tebd::set_executor(int groups) {
 // a theta with rows and columns up to 2*maxcap), as many groups as are
 // co-resident (at most `req` if req > 0, else 64), a multiple of 8 when 8 or
 // more fit.
-static void exec_shape(int maxcap, int req, int& groups, int& per_group, std::size_t& shm) {
+static void exec_shape(int maxcap, int req, bool solo, int& groups, int& per_group, std::size_t& shm) {
     const int cmax = 2 * maxcap;
     const int nb = (cmax + kBlk - 1) / kBlk;
     const int nbp = std::max(2, nb + (nb & 1));
-    per_group = nbp / 2;
+    per_group = solo ? 1 : nbp / 2;
     const std::size_t svd_bytes = (std::size_t)2 * kBlk * (2 * maxcap) * sizeof(double2);
     const std::size_t theta_bytes = (std::size_t)(kDistThreads / 256) * 2 * 16 * 2 * 17 * sizeof(double2);
-    shm = std::max(svd_bytes, theta_bytes);
+    // solo: room for a kSmall x kSmall theta of svd_small
+    const std::size_t small_bytes = solo ? (std::size_t)kSmall * kSmall * sizeof(double2) : 0;
+    shm = std::max(std::max(svd_bytes, theta_bytes), small_bytes);
     int per_cu = 0, cus = 0, dev = 0;
     TQ_HIP(hipGetDevice(&dev));
     TQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_tebd_lockstep, kDistThreads, shm));
     const long long resident = std::min<long long>((long long)per_cu * cus, kStatSlots);
     long long g = resident / per_group;
-    const long long cap = req > 0 ? req : 64;
+    const long long cap = req > 0 ? req : (solo ? kStatSlots : 64);
     g = std::min(g, cap);
     if (g >= 8) g -= g % 8;
     if (g < 1) throw Error(-3, "lockstep executor: not enough co-resident workgroups");
@@ -1540,24 +1737,32 @@ static void exec_shape(int maxcap, int req, int& groups, int& per_group, std::si
 
 void Tebd::run_exec(const double* a, const double* b, int count, int rec0) {
     std::vector<Tebd*> one{this};
-    run_lockstep(one, {a}, {b}, {count}, rec0, exec_req_);
+    run_lockstep(one, {a}, {b}, {count}, rec0, exec_req_, 1);
 }
 
 // The lockstep executor over one or several MPS (batched runs): every
 // instance's updates are placed, in its own gate order, into super-steps of at
 // most G mutually independent updates shared by all instances.
 void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const double*>& as,
-                        const std::vector<const double*>& bs, const std::vector<int>& counts, int rec0, int req) {
+                        const std::vector<const double*>& bs, const std::vector<int>& counts, int rec0, int req,
+                        int mode) {
     if (ts.empty()) return;
     if (rec0 >= 0 && ts.size() != 1) throw Error(-1, "entropy recording needs a single instance");
+    if (mode < 0 || mode > 2) throw Error(-1, "executor mode must be 0 (auto), 1 (grouped) or 2 (solo)");
     Tebd& t0 = *ts[0];
     for (Tebd* t : ts) TQ_HIP(hipStreamSynchronize(t->stream_));  // previous runs' buffers are free again
     int maxcap = 1;
     for (Tebd* t : ts) maxcap = std::max(maxcap, t->maxcap_);
     const int ld = 2 * maxcap;
+    // Grouped: each update's SVD spread over a group of workgroups in lockstep
+    // (few independent updates, large theta). Solo: one workgroup per update
+    // and one grid barrier per super-step; thetas up to 64 x 64 stay in LDS
+    // (svd_small). Automatic: solo for batches of four or more instances whose
+    // bonds stay at or below 32, where every theta is small.
+    const bool solo = mode == 2 || (mode == 0 && ts.size() >= 4 && maxcap <= 32);
     int G = 0, W = 0;
     std::size_t shm = 0;
-    exec_shape(maxcap, req, G, W, shm);
+    exec_shape(maxcap, req, solo, G, W, shm);
     using cd = std::complex<double>;
     struct M2 {
         cd u[2][2];
@@ -1572,8 +1777,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     auto to_d2 = [](cd z) { return make_double2(z.real(), z.imag()); };
     auto from_d2 = [](double2 z) { return cd(z.x, z.y); };
     const Gate4 sw = make_swap_gate();
-    std::vector<int> load;          // updates per super-step
-    std::vector<ExecUpdate> ups;    // [super-step][G]
+    std::vector<std::vector<ExecUpdate>> ups;  // per super-step, at most G
     std::vector<int> rounds;
     struct SnapAt {
         int at;
@@ -1581,10 +1785,20 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     };
     std::vector<SnapAt> snaps;
     int nsteps = 0;
-    ExecUpdate idle{};
-    idle.site = -1;
-    idle.inst = 0;
     std::vector<std::vector<M2>> pendings(ts.size());
+    // open[t]: a super-step >= t with a free slot (path-compressed), so that
+    // placing the updates of many instances stays linear in their number
+    std::vector<int> open;
+    auto first_open = [&](int t) {
+        int r = t;
+        while (r < (int)open.size() && open[r] != r) r = open[r];
+        while (t < (int)open.size() && open[t] != t) {
+            const int nx = open[t];
+            open[t] = r;
+            t = nx;
+        }
+        return r;
+    };
     for (std::size_t in = 0; in < ts.size(); ++in) {
         Tebd& T = *ts[in];
         const int n = T.n_;
@@ -1613,17 +1827,18 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
         std::vector<int> free_at(n, 0);   // first super-step at which the site may be updated
         std::vector<int> last_at(n, -1);  // super-step of the site's latest update
         auto place = [&](int p, const Gate4& g) {
-            int t = std::max(free_at[p], free_at[p + 1]);
-            while (t < (int)load.size() && load[t] >= G) ++t;
-            if (t >= (int)load.size()) {
-                load.resize(t + 1, 0);
+            const int t = first_open(std::max(free_at[p], free_at[p + 1]));
+            if (t >= (int)ups.size()) {
+                ups.resize(t + 1);
                 rounds.resize(t + 1, 0);
-                ups.resize((std::size_t)(t + 1) * G, idle);
+                for (int q = (int)open.size(); q <= t; ++q) open.push_back(q);
             }
-            ExecUpdate& u = ups[(std::size_t)t * G + load[t]++];
+            ExecUpdate u;
             u.site = p;
             u.inst = (int)in;
             u.g = g;
+            ups[t].push_back(u);
+            if ((int)ups[t].size() >= G) open[t] = t + 1;  // full: later searches skip it
             rounds[t] = std::max(rounds[t], nbp_of(p) - 1);
             free_at[p] = free_at[p + 1] = t + 1;
             nsteps = std::max(nsteps, t + 1);
@@ -1684,9 +1899,16 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     int total_steps = nsteps;
     for (const auto& x : snaps) total_steps = std::max(total_steps, x.at + 1);
     if (total_steps > (int)rounds.size()) {
-        ups.resize((std::size_t)total_steps * G, idle);
+        ups.resize(total_steps);
         rounds.resize(total_steps, 0);
     }
+    std::vector<int> up_begin(total_steps + 1, 0);
+    int widest = 1;
+    for (int t = 0; t < total_steps; ++t) {
+        up_begin[t + 1] = up_begin[t] + (int)ups[t].size();
+        widest = std::max(widest, (int)ups[t].size());
+    }
+    G = std::min(G, widest);  // launch only as many groups as a super-step uses
     std::stable_sort(snaps.begin(), snaps.end(), [](const SnapAt& x, const SnapAt& y) { return x.at < y.at; });
     std::vector<int> snap_begin(total_steps + 1, 0);
     std::vector<ExecSnap> snap_list(snaps.size());
@@ -1709,8 +1931,8 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
         t0.ex_sv_.alloc((std::size_t)G * ld);
         t0.ex_perm_.alloc((std::size_t)G * ld);
         t0.ex_taus_.alloc((std::size_t)G * ld);
-        t0.ex_flags_.alloc(4 * (std::size_t)G + 2);
     }
+    if (t0.ex_flags_.n < 4 * (std::size_t)G + 2) t0.ex_flags_.alloc(4 * (std::size_t)G + 2);
     const int count0 = counts[0];
     if (rec0 >= 0 && t0.ex_snap_.n < (std::size_t)count0 * t0.off_[t0.n_]) {
         t0.ex_snap_.alloc((std::size_t)count0 * t0.off_[t0.n_]);
@@ -1723,18 +1945,22 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     }
     auto align = [](std::size_t x) { return (x + 255) & ~(std::size_t)255; };
     const std::size_t in_bytes = align(insts.size() * sizeof(ExecInst));
-    const std::size_t up_bytes = align(ups.size() * sizeof(ExecUpdate));
+    const std::size_t up_bytes = align(std::max<std::size_t>(1, up_begin[total_steps]) * sizeof(ExecUpdate));
     const std::size_t sn_bytes = align(std::max<std::size_t>(1, snap_list.size()) * sizeof(ExecSnap));
-    const std::size_t int_count = (std::size_t)total_steps + (total_steps + 1);
+    const std::size_t int_count = (std::size_t)total_steps + 2 * (total_steps + 1);
     t0.ex_host_.assign(in_bytes + up_bytes + sn_bytes + int_count * sizeof(int), 0);
     unsigned char* hb = t0.ex_host_.data();
     std::memcpy(hb, insts.data(), insts.size() * sizeof(ExecInst));
-    std::memcpy(hb + in_bytes, ups.data(), ups.size() * sizeof(ExecUpdate));
+    for (int t = 0; t < total_steps; ++t)
+        if (!ups[t].empty())
+            std::memcpy(hb + in_bytes + (std::size_t)up_begin[t] * sizeof(ExecUpdate), ups[t].data(),
+                        ups[t].size() * sizeof(ExecUpdate));
     if (!snap_list.empty())
         std::memcpy(hb + in_bytes + up_bytes, snap_list.data(), snap_list.size() * sizeof(ExecSnap));
     int* hi = reinterpret_cast<int*>(hb + in_bytes + up_bytes + sn_bytes);
     std::memcpy(hi, rounds.data(), total_steps * sizeof(int));
     std::memcpy(hi + total_steps, snap_begin.data(), (total_steps + 1) * sizeof(int));
+    std::memcpy(hi + 2 * total_steps + 1, up_begin.data(), (total_steps + 1) * sizeof(int));
     if (t0.ex_tasks_.n < t0.ex_host_.size()) t0.ex_tasks_.alloc(t0.ex_host_.size());
     TQ_HIP(hipMemcpy(t0.ex_tasks_.p, hb, t0.ex_host_.size(), hipMemcpyHostToDevice));
 
@@ -1744,6 +1970,9 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.snaps = reinterpret_cast<const ExecSnap*>(t0.ex_tasks_.p + in_bytes + up_bytes);
     x.rounds = reinterpret_cast<const int*>(t0.ex_tasks_.p + in_bytes + up_bytes + sn_bytes);
     x.snap_begin = x.rounds + total_steps;
+    x.up_begin = x.rounds + 2 * total_steps + 1;
+    x.solo = solo ? 1 : 0;
+    x.small = solo ? 1 : 0;
     x.nsteps = total_steps;
     x.G = G;
     x.n = t0.n_;
@@ -1775,7 +2004,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
                                           args, shm, st));
     if (t0.timing_) TQ_HIP(hipEventRecord(t0.events_[t0.ev_next_++].second, st));
     t0.exec_supersteps_ += total_steps;
-    for (int t = 0; t < (int)load.size(); ++t) t0.exec_updates_ += load[t];
+    t0.exec_updates_ += up_begin[total_steps];
     // fold the single-site gates still pending into the sites
     std::vector<std::vector<double2>> gates(ts.size());
     for (std::size_t in = 0; in < ts.size(); ++in) {
@@ -1834,9 +2063,22 @@ void Tebd::entropy_sweep(int rec) { entropy_of(sites_.p, dims_.p, rec); }
 
 void Tebd::entropy_of(const double2* sites, const int* dims, int rec) {
     ensure_trace(rec + 1);
+    entropy_into(sites, dims, trace_.p + rec);
+}
+
+double Tebd::entropy_now() {
+    DevBuf<double> out(1);
+    entropy_into(sites_.p, dims_.p, out.p);
+    double s = 0.0;
+    TQ_HIP(hipMemcpyAsync(&s, out.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    return s;
+}
+
+void Tebd::entropy_into(const double2* sites, const int* dims, double* dst) {
     if (n_ < 2) {
         const double zero = 0.0;
-        TQ_HIP(hipMemcpyAsync(trace_.p + rec, &zero, sizeof(double), hipMemcpyHostToDevice, stream_));
+        TQ_HIP(hipMemcpyAsync(dst, &zero, sizeof(double), hipMemcpyHostToDevice, stream_));
         TQ_HIP(hipStreamSynchronize(stream_));
         return;
     }
@@ -1849,7 +2091,7 @@ void Tebd::entropy_of(const double2* sites, const int* dims, int rec) {
         std::swap(cur, nxt);
     }
     hipLaunchKernelGGL(k_rho0, dim3(1), dim3(64), 0, stream_, sites, off_dev_.p, caps_dev_.p, dims, cur,
-                       chi_max(), trace_.p, rec);
+                       chi_max(), dst, 0);
     TQ_LAUNCH_CHECK();
 }
 
@@ -1977,7 +2219,20 @@ Tebd::Readout Tebd::readout() {
     TQ_HIP(hipMemcpyAsync(&amp, res.p + n_ + m + 1, sizeof(double2), hipMemcpyDeviceToHost, stream_));
     sync();
     out.prob = nrm > 0 ? (amp.x * amp.x + amp.y * amp.y) / nrm : 0.0;
+    out.z = std::move(z);
     return out;
+}
+
+void Tebd::set_bias(const double* bias) {
+    sync();  // launches already queued may still read the old field
+    if (!bias) {
+        model_.bias.clear();
+        bias_.release();
+        return;
+    }
+    model_.bias.assign(bias, bias + n_);
+    if (!bias_.p) bias_.alloc(n_);
+    TQ_HIP(hipMemcpy(bias_.p, model_.bias.data(), n_ * sizeof(double), hipMemcpyHostToDevice));
 }
 
 // theta of sites (p, p+1) under the identity gate, as the SVD sees it: R x C

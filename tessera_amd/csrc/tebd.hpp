@@ -42,7 +42,9 @@ public:
     Tebd(const Tebd&) = delete;
     Tebd& operator=(const Tebd&) = delete;
 
-    void reset_product();                       // |+>^n
+    // |+>^n, or with `warm` the product of the single-spin ground states of
+    // -(X_i + warm_i Z_i) (the s = 0 ground state under a warm-start bias).
+    void reset_product(const double* warm = nullptr);
     // Random MPS with every bond at capacity, then one left-to-right sweep of
     // identity updates giving bond weights ~exp(-k/decay) (a saturated state).
     void randomize(uint64_t seed, double decay);
@@ -58,10 +60,18 @@ public:
     int executor_groups() const { return exec_req_; }
     // Batched lockstep run: instance i advances counts[i] steps with schedule
     // values as[i][k], bs[i][k]; entropy recording (rec0 >= 0) needs one instance.
+    // mode: 0 auto (solo for four or more instances with chi <= 32), 1 grouped
+    // (an update's SVD spread over a group of workgroups), 2 solo (one
+    // workgroup per update).
     static void run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const double*>& as,
-                             const std::vector<const double*>& bs, const std::vector<int>& counts, int rec0, int req);
+                             const std::vector<const double*>& bs, const std::vector<int>& counts, int rec0, int req,
+                             int mode = 0);
     void sync();
     std::vector<double> entropy(int count);      // recorded slots [0, count)
+    double entropy_now();                        // qubit-0 entropy of the current state (mps.cpp:218-291)
+    // Per-spin longitudinal field added to the single-site generator from the
+    // next step on: -A X - (B h_i + A bias_i) Z; nullptr removes it.
+    void set_bias(const double* bias);
     double max_trunc();
     SvdStats svd_stats();
     void set_timing(bool on);  // HIP events around every SVD launch
@@ -69,6 +79,7 @@ public:
     struct Readout {
         std::vector<int8_t> spins;
         double energy = 0, expected_energy = 0, prob = 0;
+        std::vector<double> z;  // <Z_i> / <psi|psi>
     };
     Readout readout();                           // mps.cpp:358-380
     std::vector<double> dense();                 // n <= 24, interleaved re/im
@@ -86,6 +97,7 @@ private:
     void two_site(int p, const Gate4& g, const double* wts = nullptr);
     void entropy_sweep(int rec);
     void entropy_of(const double2* sites, const int* dims, int rec);
+    void entropy_into(const double2* sites, const int* dims, double* dst);
     void env_left(const double2* lin, int site, bool z, double2* lout);
     void env_right(const double2* rin, int site, double2* rout, const double2* sites = nullptr,
                    const int* dims = nullptr);

@@ -83,8 +83,18 @@ def lib() -> ctypes.CDLL:
         L.tessera_tebd_theta.argtypes = [vp, _usz, vp, _usz, vp, vp]
         L.tessera_tebd_batch_run.restype = ctypes.c_int
         L.tessera_tebd_batch_run.argtypes = [vp, _usz, vp, vp, vp, ctypes.c_int]
+        L.tessera_tebd_batch_run_ex.restype = ctypes.c_int
+        L.tessera_tebd_batch_run_ex.argtypes = [vp, _usz, vp, vp, vp, ctypes.c_int, ctypes.c_int]
         L.tessera_tebd_exec_stats.restype = ctypes.c_int
         L.tessera_tebd_exec_stats.argtypes = [vp, vp, vp]
+        L.tessera_tebd_reset_warm.restype = ctypes.c_int
+        L.tessera_tebd_reset_warm.argtypes = [vp, vp]
+        L.tessera_tebd_expect_z.restype = ctypes.c_int
+        L.tessera_tebd_expect_z.argtypes = [vp, vp]
+        L.tessera_tebd_entropy_now.restype = ctypes.c_int
+        L.tessera_tebd_entropy_now.argtypes = [vp, vp]
+        L.tessera_tebd_set_bias.restype = ctypes.c_int
+        L.tessera_tebd_set_bias.argtypes = [vp, vp]
         L.tessera_tebd_updates_per_step.restype = _usz
         L.tessera_tebd_updates_per_step.argtypes = [vp]
         L.tessera_tebd_destroy.restype = None
@@ -263,6 +273,14 @@ class TebdSession:
         _check(lib().tessera_tebd_reset(self._h))
         self.recorded = 0
 
+    def reset_warm(self, bias):
+        """Product ground state of -sum_i (X_i + bias_i Z_i) (None: |+>^n)."""
+        arr = None if bias is None else np.ascontiguousarray(bias, np.float64)
+        if arr is not None and arr.shape != (self.n,):
+            raise ValueError(f"bias must have {self.n} entries")
+        _check(lib().tessera_tebd_reset_warm(self._h, _p(arr)))
+        self.recorded = 0
+
     def randomize(self, seed: int, decay: float = 40.0):
         _check(lib().tessera_tebd_randomize(self._h, seed, decay))
         self.recorded = 0
@@ -286,6 +304,26 @@ class TebdSession:
         sc = np.zeros(4)
         _check(lib().tessera_tebd_readout(self._h, _p(spins), _p(sc)))
         return spins, sc
+
+    def expect_z(self) -> np.ndarray:
+        """<Z_i> of the current state (normalised), the values the readout signs."""
+        z = np.zeros(self.n)
+        _check(lib().tessera_tebd_expect_z(self._h, _p(z)))
+        return z
+
+    def entropy_now(self) -> float:
+        """Qubit-0 entanglement entropy of the current state."""
+        s = ctypes.c_double()
+        _check(lib().tessera_tebd_entropy_now(self._h, ctypes.addressof(s)))
+        return s.value
+
+    def set_bias(self, bias):
+        """Longitudinal warm-start field -A(s) sum_i bias_i Z_i for the steps run from
+        now on (None: the plain anneal)."""
+        arr = None if bias is None else np.ascontiguousarray(bias, np.float64)
+        if arr is not None and arr.shape != (self.n,):
+            raise ValueError(f"bias must have {self.n} entries")
+        _check(lib().tessera_tebd_set_bias(self._h, _p(arr)))
 
     def dense(self) -> np.ndarray:
         out = np.zeros(2 << self.n)
@@ -334,9 +372,11 @@ class TebdSession:
         return int(lib().tessera_tebd_updates_per_step(self._h))
 
 
-def batch_run(sessions, schedules, groups: int = -1):
-    """Advance several TebdSession objects together (tessera_tebd_batch_run):
-    session i by the steps of schedules[i] (an object with .a and .b arrays)."""
+def batch_run(sessions, schedules, groups: int = -1, mode: int = 0):
+    """Advance several TebdSession objects together (tessera_tebd_batch_run_ex):
+    session i by the steps of schedules[i] (an object with .a and .b arrays).
+    mode 0 = automatic layout, 1 = grouped workgroups per update, 2 = one
+    workgroup per update."""
     k = len(sessions)
     if k == 0:
         return
@@ -353,5 +393,5 @@ def batch_run(sessions, schedules, groups: int = -1):
         b_ptrs[i] = b.ctypes.data
         steps[i] = a.shape[0]
         handles[i] = s._h
-    _check(lib().tessera_tebd_batch_run(ctypes.addressof(handles), k, ctypes.addressof(a_ptrs),
-                                        ctypes.addressof(b_ptrs), _p(steps), int(groups)))
+    _check(lib().tessera_tebd_batch_run_ex(ctypes.addressof(handles), k, ctypes.addressof(a_ptrs),
+                                           ctypes.addressof(b_ptrs), _p(steps), int(groups), int(mode)))

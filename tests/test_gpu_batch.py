@@ -27,7 +27,10 @@ def _mixed():
     ]
 
 
-def test_batch_matches_one_at_a_time():
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_batch_matches_one_at_a_time(mode):
+    """mode 0: automatic layout (grouped here: bonds up to 64), 1: grouped, 2: solo
+    (thetas up to 64 x 64 in LDS, larger ones through the block SVD)."""
     cases = _mixed()
     solo = []
     for model, sched, dt, chi in cases:
@@ -38,7 +41,7 @@ def test_batch_matches_one_at_a_time():
         solo.append((s.dense(), tr, bonds, spins, sc))
         s.close()
     sessions = [qa.TebdSession(model, chi, dt) for model, _, dt, chi in cases]
-    qa.batch_run(sessions, [c[1] for c in cases])
+    qa.batch_run(sessions, [c[1] for c in cases], mode=mode)
     for k, s in enumerate(sessions):
         _, tr, bonds = s.sync()
         spins, sc = s.readout()
@@ -53,7 +56,8 @@ def test_batch_matches_one_at_a_time():
         s.close()
 
 
-def test_batch_continues_and_groups_agree():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_continues_and_groups_agree(mode):
     """Two consecutive batch calls equal one, and the group count does not
     change the result (1 group: strictly sequential super-steps)."""
     cases = _mixed()[:3]
@@ -63,10 +67,10 @@ def test_batch_continues_and_groups_agree():
         if split:
             first = [I.Schedule(c[1].a[:4], c[1].b[:4]) for c in cases]
             rest = [I.Schedule(c[1].a[4:], c[1].b[4:]) for c in cases]
-            qa.batch_run(sessions, first, groups)
-            qa.batch_run(sessions, rest, groups)
+            qa.batch_run(sessions, first, groups, mode)
+            qa.batch_run(sessions, rest, groups, mode)
         else:
-            qa.batch_run(sessions, [c[1] for c in cases], groups)
+            qa.batch_run(sessions, [c[1] for c in cases], groups, mode)
         out.append([s.dense() for s in sessions])
         for s in sessions:
             s.close()
@@ -77,15 +81,30 @@ def test_batch_continues_and_groups_agree():
 def test_batch_config4_sample_runs():
     """Eight config-4 instances (n up to 96, chi in {32, 64, 128}) for three
     steps: the batched run completes and every session reads out finite values
-    with bonds within chi."""
+    with bonds within chi; the solo and grouped layouts agree on the expected
+    energy and discard nothing. Their ranks may differ: three steps from the
+    product state leave many singular values at the rounding level of theta
+    (~1e-14 of its norm), and the two SVD variants (in-LDS pair Jacobi for
+    small thetas, LQ-preconditioned block Jacobi) round them to different sides
+    of the S < 1e-14 keep cut (mps.cpp:77-85); what either drops weighs below
+    1e-20. (On the mixed exact cases above the ranks are identical.)"""
     insts = I.config4(8)
-    sessions = [qa.TebdSession(x.model, x.chi, x.dt) for x in insts]
     scheds = [I.Schedule(x.sched.a[:3], x.sched.b[:3]) for x in insts]
-    qa.batch_run(sessions, scheds)
-    for x, s in zip(insts, sessions):
-        _, tr, bonds = s.sync()
-        spins, sc = s.readout()
-        assert np.all(np.isfinite(sc))
-        assert bonds.max() <= x.chi
-        assert set(np.unique(spins)) <= {-1, 1}
-        s.close()
+    res = []
+    for mode in (2, 1):
+        sessions = [qa.TebdSession(x.model, x.chi, x.dt) for x in insts]
+        qa.batch_run(sessions, scheds, mode=mode)
+        out = []
+        for x, s in zip(insts, sessions):
+            _, tr, bonds = s.sync()
+            spins, sc = s.readout()
+            assert np.all(np.isfinite(sc))
+            assert bonds.max() <= x.chi
+            assert set(np.unique(spins)) <= {-1, 1}
+            out.append((bonds, sc[1], tr))
+            s.close()
+        res.append(out)
+    for (b2, e2, t2), (b1, e1, t1) in zip(*res):
+        print(f"rank difference solo - grouped: max {np.abs(b2.astype(int) - b1).max()}", flush=True)
+        assert abs(e2 - e1) <= 1e-9
+        assert t2 <= 1e-20 and t1 <= 1e-20

@@ -24,9 +24,10 @@ from tessera_amd import qa
 pytestmark = pytest.mark.gpu
 
 
-def dense_mps_splitting(model, sched, dt):
+def dense_mps_splitting(model, sched, dt, bias=None, psi0=None):
     """Exact evolution of the MPS engine's splitting U1(dt/2) ZZ(dt) U1(dt/2)
-    (mps.cpp:309-356), physical index 0 <-> +1 stored in bit i."""
+    (mps.cpp:309-356), physical index 0 <-> +1 stored in bit i. With bias the
+    single-site generator is -A X - (B h_i + A bias_i) Z; psi0 replaces |+>^n."""
     n = model.n
     dim = 1 << n
     x = np.arange(dim)
@@ -34,12 +35,14 @@ def dense_mps_splitting(model, sched, dt):
     zz = np.zeros(dim)
     for i, j, w in zip(model.ei, model.ej, model.ew):
         zz += w * spins[int(i)] * spins[int(j)]
-    psi = np.ones(dim, complex) / math.sqrt(dim)
+    psi = np.ones(dim, complex) / math.sqrt(dim) if psi0 is None else np.asarray(psi0, complex).copy()
+    bias = np.zeros(n) if bias is None else np.asarray(bias, float)
     ents = []
 
     def layer(psi, A, B):
         for i in range(n):
-            g = np.array([[-B * model.h[i], -A], [-A, B * model.h[i]]])
+            hz = B * model.h[i] + A * bias[i]
+            g = np.array([[-hz, -A], [-A, hz]])
             w, V = np.linalg.eigh(g)
             U = V @ np.diag(np.exp(-1j * dt / 2 * w)) @ V.conj().T
             psi = np.einsum("ab,xbz->xaz", U, psi.reshape(-1, 2, 1 << i)).reshape(dim)

@@ -770,9 +770,13 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
                           double* __restrict__ sv_out, int* __restrict__ perm_out, double* trunc,
                           unsigned long long* stats, double2* A, SvdShared& sh) {
     const int tid = threadIdx.x, l16 = tid & 15, grp = tid >> 4, ngrp = blockDim.x >> 4;
-    for (int idx = tid; idx < m * c; idx += blockDim.x) {
-        const int col = idx / m, row = idx % m;
-        A[idx] = Wg[(std::size_t)col * ld + row];
+    // columns enter LDS in order of decreasing norm (fewer sweeps); LDS column q
+    // holds working column sh.perm[q] and goes back to that column at the end
+    for (int col = grp; col < c; col += ngrp) {
+        double s2 = 0.0;
+        for (int row = l16; row < m; row += 16) s2 += cnorm(Wg[(std::size_t)col * ld + row]);
+        s2 = sum16(s2);
+        if (l16 == 0) sh.nrm[col] = s2;
     }
     if (tid == 0) {
         sh.vis = 0;
@@ -780,12 +784,26 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
     }
     __syncthreads();
     double fro = 0.0;
-    for (int idx = tid; idx < m * c; idx += blockDim.x) fro += cnorm(A[idx]);
+    for (int q = tid; q < c; q += blockDim.x) {
+        const double v = sh.nrm[q];
+        fro += v;
+        int rank = 0;
+        for (int o = 0; o < c; ++o) {
+            const double u = sh.nrm[o];
+            rank += (u > v) || (u == v && o < q);
+        }
+        sh.perm[rank] = q;
+    }
     fro = wave_sum(fro);
     if ((tid & 63) == 0) sh.red[tid >> 6][0].x = fro;
     __syncthreads();
+    for (int idx = tid; idx < m * c; idx += blockDim.x) {
+        const int q = idx / m, row = idx % m;
+        A[idx] = Wg[(std::size_t)sh.perm[q] * ld + row];
+    }
     fro = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) fro += sh.red[w][0].x;
+    __syncthreads();
     const double floor2 = noise_floor2(fro), tol = jacobi_tol(m);
     const int cp = c + (c & 1);
     int sweeps = 0;
@@ -821,8 +839,9 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
                 if (al <= floor2 || be <= floor2) continue;
                 const double gabs = sqrt(gr * gr + gi * gi);
                 if (!(gabs > 0.0) || gabs <= tol * sqrt(al * be)) continue;
-                const double2 ph = make_double2(gr / gabs, -gi / gabs);
-                const double zeta = (be - al) / (2.0 * gabs);
+                const double ig = 1.0 / gabs;
+                const double2 ph = make_double2(gr * ig, -gi * ig);
+                const double zeta = (be - al) * (0.5 * ig);
                 const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                 const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
 #pragma unroll
@@ -843,20 +862,21 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
         __syncthreads();
         if (!f) break;
     }
+    // the rotated columns go back to the working copy the reconstruction reads
+    // (LDS column q to working column q: the ordering below refers to these)
+    for (int idx = tid; idx < m * c; idx += blockDim.x) {
+        const int col = idx / m, row = idx % m;
+        Wg[(std::size_t)col * ld + row] = A[idx];
+    }
     for (int col = grp; col < c; col += ngrp) {
-        double s = 0.0;
-        for (int row = l16; row < m; row += 16) s += cnorm(A[(std::size_t)col * m + row]);
-        s = sum16(s);
-        if (l16 == 0) sh.nrm[col] = sqrt(s);
+        double s2 = 0.0;
+        for (int row = l16; row < m; row += 16) s2 += cnorm(A[(std::size_t)col * m + row]);
+        s2 = sum16(s2);
+        if (l16 == 0) sh.nrm[col] = sqrt(s2);
     }
     if (l16 == 0 && nvis) {
         atomicAdd(&sh.vis, nvis * (unsigned long long)m);  // LDS
         atomicAdd(&sh.rots, nrot * (unsigned long long)m);
-    }
-    // the rotated columns go back to the working copy the reconstruction reads
-    for (int idx = tid; idx < m * c; idx += blockDim.x) {
-        const int col = idx / m, row = idx % m;
-        Wg[(std::size_t)col * ld + row] = A[idx];
     }
     __syncthreads();
     for (int q = tid; q < c; q += blockDim.x) {  // stable descending rank

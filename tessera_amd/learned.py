@@ -214,6 +214,7 @@ class Targets:
     energy: np.ndarray  # readout energy of the plain anneal
     entropy: list      # per instance: entropy at the end of every chunk
     seconds: float
+    updates: int = 0   # two-site updates the anneals performed
 
 
 def _chunks(steps: int, chunks: int):
@@ -227,10 +228,12 @@ def tebd_targets(instances: list[Instance], chunks: int = 10, group: int = 128, 
     sampled at the end of each of `chunks` schedule segments. `log` (optional)
     receives a line per finished chunk."""
     z, sp, en, ents = [], [], [], []
+    updates = 0
     t0 = time.perf_counter()
     for k in range(0, len(instances), group):
         part = instances[k:k + group]
         sess = [TebdSession(x.model, x.chi, x.dt) for x in part]
+        updates += sum(s.updates_per_step * x.sched.steps for s, x in zip(sess, part))
         spans = [_chunks(x.sched.steps, chunks) for x in part]
         trace = [[] for _ in part]
         for c in range(chunks):
@@ -253,7 +256,7 @@ def tebd_targets(instances: list[Instance], chunks: int = 10, group: int = 128, 
             sp.append((peak_end - 1) / max(part[i].sched.steps - 1, 1))
             s.close()
     return Targets(z=z, s_peak=np.asarray(sp), energy=np.asarray(en), entropy=ents,
-                   seconds=time.perf_counter() - t0)
+                   seconds=time.perf_counter() - t0, updates=updates)
 
 
 # ---------------------------------------------------------------- guidance

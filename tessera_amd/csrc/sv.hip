@@ -187,6 +187,9 @@ __global__ void __launch_bounds__(1024) k_sv_resident(Model md, const double* sa
 // transcendental.
 constexpr int kMaxTerms = 256;
 constexpr int kMaxCrossT = 64;
+// Register phase block: up to this many L-H couplings (their 2^k sign patterns
+// are tabulated once per pass); more take the LDS phase block.
+constexpr int kFastCross = 6;
 
 struct PassArgs {
     int c, lo;         // tile: 2^c contiguous amplitudes, window at lo (lo < 0: contiguous 4096 block)
@@ -196,6 +199,11 @@ struct PassArgs {
     int phase_out, phase_in;
     double scale_out, scale_in;
     int best;          // track the argmax (last step)
+    int fast;          // register phase block (L-H couplings <= kFastCross)
+    int nx;            // L-H couplings: local bits xa[q] (L), xb[q] (H), weight xw[q]
+    int xa[kFastCross], xb[kFastCross];
+    double xw[kFastCross];
+    int x0mask;        // couplings that involve local bit 0 (flip with it)
 };
 
 __device__ inline double bit_spin(unsigned long long x, int b) { return ((x >> b) & 1ull) ? 1.0 : -1.0; }
@@ -315,10 +323,25 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
     __shared__ double2 cstPo, cstPi;
     __shared__ double red[kThreads / 64][kObs + 1];
     __shared__ long long redx[kThreads / 64];
+    __shared__ double xE[1 << kFastCross];
     const int t = threadIdx.x;
+    // fast path: the phase of one application covers phase_out and phase_in
+    const double s_c = (pa.phase_out ? pa.scale_out : 0.0) + (pa.phase_in ? pa.scale_in : 0.0);
     if (PH) {
         for (int q = t; q < pt.off[kKinds]; q += kThreads) tm[q] = pt.t[q];
         if (t <= kKinds) off[t] = pt.off[t];
+        if (pa.fast && t < (1 << pa.nx)) {
+            // pass constants: energy and phasors of the L-H couplings per sign pattern
+            // (bit q of t set <-> the spins of coupling q differ: -w s s = +w)
+            double e = 0.0;
+            for (int q = 0; q < pa.nx; ++q) e += ((t >> q) & 1) ? pa.xw[q] : -pa.xw[q];
+            xE[t] = e;
+            double sn, cs;
+            sincos(s_c * e, &sn, &cs);
+            crossPo[t] = make_double2(cs, -sn);  // exp(-i s_c e)
+            sincos(pa.scale_out * e, &sn, &cs);
+            crossPi[t] = make_double2(cs, -sn);  // exp(-i s_out e)
+        }
         __syncthreads();
     }
     const int c = pa.c, lo = pa.lo;
@@ -346,8 +369,23 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
         if (PH) {
             build_energy(tm, off, base, tabE, &cst);
             __syncthreads();
-            if (pa.phase_out) build_phasors(tm, off, pa.scale_out, tabE, cst, tabPo, crossPo, &cstPo);
-            if (pa.phase_in) build_phasors(tm, off, pa.scale_in, tabE, cst, tabPi, crossPi, &cstPi);
+            if (pa.fast) {
+                // tabPo = exp(-i s_c T), tabPi[0..63] = exp(-i s_out T1), cstPo = exp(-i s_c C)
+                double sn, cs;
+                if (t < 128) {
+                    sincos(s_c * tabE[t], &sn, &cs);
+                    tabPo[t] = make_double2(cs, -sn);
+                } else if (t < 192) {
+                    sincos(pa.scale_out * tabE[t - 128], &sn, &cs);
+                    tabPi[t - 128] = make_double2(cs, -sn);
+                } else if (t == 192) {
+                    sincos(s_c * cst, &sn, &cs);
+                    cstPo = make_double2(cs, -sn);
+                }
+            } else {
+                if (pa.phase_out) build_phasors(tm, off, pa.scale_out, tabE, cst, tabPo, crossPo, &cstPo);
+                if (pa.phase_in) build_phasors(tm, off, pa.scale_in, tabE, cst, tabPi, crossPi, &cstPi);
+            }
             __syncthreads();
         }
         // two rounds: [rotate(k)] [phase block] / [rotate(k+1)]; one copy of the rotation code
@@ -355,7 +393,56 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
         for (int round = 0; round < 2; ++round) {
         if (round == 0 ? pa.rot_a : pa.rot_b)
             rotate_group(v, sh, pa.mask, round == 0 ? pa.ca : pa.cb, round == 0 ? pa.sa : pa.sb);
-        if (PH && round == 0) {
+        if (PH && round == 0 && pa.fast) {
+            // phase block, one sweep. In lay0 amplitude j = t + 256 r: its L bits
+            // (j & 63) are fixed per thread and its H bits are j >> 6, so
+            // E(j) = C + T1[t & 63] + T2[j >> 6] + xE[k(j)] with k(j) the sign
+            // pattern of the L-H couplings; one phasor exp(-i s_c E) applies
+            // phase_out and phase_in together. rho01 pairs (j, j ^ 1) sit in lanes
+            // (t, t ^ 1) and need the phase_out factor Po(j) conj(Po(j ^ 1)), in
+            // which C and T2 cancel: P1o[lo] conj(P1o[lo ^ 1]) xPo[k] conj(xPo[k ^ x0mask]).
+            // The tile goes through each thread's own LDS slots (no barrier: a
+            // thread reads back only what it wrote) so the loop need not be unrolled.
+            const int lo6 = t & 63;
+            const double e_t = cst + tabE[lo6];
+            const double2 pc_t = cmul(cstPo, tabPo[lo6]);
+            const bool even = (t & 1) == 0;
+            double2 r1 = make_double2(1.0, 0.0);
+            if (pa.phase_out) r1 = cmul(tabPi[lo6], cconj(tabPi[lo6 ^ 1]));
+#pragma unroll
+            for (int r = 0; r < kRegs; ++r) sh[sidx(lay0(t, r))] = v[r];
+#pragma unroll 1
+            for (int r = 0; r < kRegs; ++r) {
+                const int j = t + (r << 8);
+                int k = 0;
+#pragma unroll
+                for (int q = 0; q < kFastCross; ++q)
+                    if (q < pa.nx) k |= (((j >> pa.xa[q]) ^ (j >> pa.xb[q])) & 1) << q;
+                const int hi = j >> 6;
+                const double2 u = sh[sidx(j)];
+                if (pa.phase_out) {
+                    const double p = u.x * u.x + u.y * u.y;
+                    const double e = e_t + tabE[64 + hi] + xE[k];
+                    acc[4] += p;
+                    acc[5] += p * e;
+                    if (pa.best) best = better(best, Best{p, (long long)gidx(j)});
+                    const double2 w = make_double2(__shfl_xor(u.x, 1, 64), __shfl_xor(u.y, 1, 64));
+                    if (even) {
+                        const double2 corr = cmul(r1, cmul(crossPi[k], cconj(crossPi[k ^ pa.x0mask])));
+                        const double2 uw = make_double2(u.x * w.x + u.y * w.y, u.y * w.x - u.x * w.y);
+                        const double2 z = cmul(uw, corr);
+                        acc[0] += p;
+                        acc[1] += w.x * w.x + w.y * w.y;
+                        acc[2] += z.x;
+                        acc[3] += z.y;
+                    }
+                }
+                sh[sidx(j)] = cmul(u, cmul(pc_t, cmul(tabPo[64 + hi], crossPo[k])));
+            }
+#pragma unroll
+            for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay0(t, r))];
+        }
+        if (PH && round == 0 && !pa.fast) {
             // phase block on the LDS copy of the tile (lay0 order), plain loops keep registers low
 #pragma unroll
             for (int r = 0; r < kRegs; ++r) sh[sidx(lay0(t, r))] = v[r];
@@ -583,6 +670,7 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
     } else {
         const long long ntiles = dim >> kTileBits;
         const int nblocks = (int)std::min<long long>(ntiles, 4096);
+        const bool fast_phase = true;  // register phase block wherever the L-H couplings fit
         DevBuf<double> partials((std::size_t)nblocks * kObs);
         DevBuf<double> bp(nblocks);
         DevBuf<long long> bx(nblocks);
@@ -664,6 +752,18 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
             pa.c = g == 0 ? 0 : groups[g].c;
             pa.lo = groups[g].lo;
             pa.mask = groups[g].mask;
+            const int nlh = pts[g].off[kLH + 1] - pts[g].off[kLH];
+            if (fast_phase && nlh <= kFastCross) {
+                pa.fast = 1;
+                pa.nx = nlh;
+                for (int q = 0; q < nlh; ++q) {
+                    const Term& tq = terms[starts[g] + pts[g].off[kLH] + q];
+                    pa.xa[q] = tq.a;
+                    pa.xb[q] = tq.b;
+                    pa.xw[q] = tq.w;
+                    if (tq.a == 0 || tq.b == 0) pa.x0mask |= 1 << q;
+                }
+            }
             return pa;
         };
         for (int k = 0; k < pb.steps; ++k) {

@@ -55,6 +55,14 @@ def test_tiled_passes_match_oracle(n, steps):
     _compare(model, I.linear(steps), 0.05)
 
 
+def test_dense_couplings_take_lds_phase_block():
+    """Range-8 chain: more than six couplings between local bits 0-5 and 6-11, so
+    the merged pass keeps the LDS phase block; the sparse cases above take the
+    register phase block."""
+    model = I.chain_range(15, 8, 115)
+    _compare(model, I.linear(10), 0.05)
+
+
 def test_single_spin_closed_form():
     # n = 1: exact 2x2 evolution; quantum.rs:229-237 expectations
     model = I.make_ising([1.0], [])

@@ -282,37 +282,51 @@ __device__ inline double2 tile_phasor(const Term* tm, const int* off, int j, con
     return f;
 }
 
-// Rotates the group bits of a tile held as v[r] in layout lay0: exchanges cycle
-// local bits 0-3, 4-7, 8-11 into registers; ends in lay0.
-__device__ inline void rotate_group(double2 (&v)[kRegs], double2* sh, unsigned mask, double c, double s) {
+// Layout of a tile in registers: which local bits the 16 register slots hold.
+enum { kLay0 = 0, kLay1 = 1 };
+__device__ inline int lay_of(int which, int t, int r) { return which == kLay0 ? lay0(t, r) : lay1(t, r); }
+
+// One full-tile exchange through LDS from layout `from` to layout `to`.
+template <int FROM, int TO>
+__device__ inline void exchange(double2 (&v)[kRegs], double2* sh) {
     const int t = threadIdx.x;
+    auto lay = [&](int which, int r) { return which == 0 ? lay0(t, r) : which == 1 ? lay1(t, r) : lay2(t, r); };
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) sh[sidx(lay0(t, r))] = v[r];
+    for (int r = 0; r < kRegs; ++r) sh[sidx(lay(FROM, r))] = v[r];
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay2(t, r))];
+    for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay(TO, r))];
     __syncthreads();
+}
+
+// Rotates the group bits of a tile with two exchanges: from lay0 (registers hold
+// bits 8-11) through lay2 (bits 0-3) to lay1 (bits 4-7), or back from lay1 to
+// lay0. The qubits of a group commute; within a group the order differs from
+// the upstream ascending loop only by rounding.
+__device__ inline void rotate_group_fwd(double2 (&v)[kRegs], double2* sh, unsigned mask, double c, double s) {
+    rotate_regs(v, (mask >> 8) & 15u, c, s);
+    exchange<0, 2>(v, sh);
     rotate_regs(v, mask & 15u, c, s);
-#pragma unroll
-    for (int r = 0; r < kRegs; ++r) sh[sidx(lay2(t, r))] = v[r];
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay1(t, r))];
-    __syncthreads();
+    exchange<2, 1>(v, sh);
     rotate_regs(v, (mask >> 4) & 15u, c, s);
-#pragma unroll
-    for (int r = 0; r < kRegs; ++r) sh[sidx(lay1(t, r))] = v[r];
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay0(t, r))];
-    __syncthreads();
+}
+__device__ inline void rotate_group_bwd(double2 (&v)[kRegs], double2* sh, unsigned mask, double c, double s) {
+    rotate_regs(v, (mask >> 4) & 15u, c, s);
+    exchange<1, 2>(v, sh);
+    rotate_regs(v, mask & 15u, c, s);
+    exchange<2, 0>(v, sh);
     rotate_regs(v, (mask >> 8) & 15u, c, s);
 }
 
-template <bool PH>
+// PH: phase block (merged pass); RA / RB: rotate before / after it. The tile is
+// loaded in lay0; a rotation from lay0 ends in lay1 and one from lay1 in lay0,
+// so the phase layout and the store layout follow from RA and RB at compile time.
+template <bool PH, bool RA, bool RB>
 __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ psi, PassTerms pt, PassArgs pa,
                                                       long long ntiles, double* __restrict__ partials,
                                                       double* __restrict__ best_p, long long* __restrict__ best_x) {
+    constexpr int cur = RA ? kLay1 : kLay0;                         // layout of the phase block
+    constexpr int out_lay = RB ? (cur == kLay1 ? kLay0 : kLay1) : cur;  // layout of the store
     __shared__ double2 sh[kTile + kTile / 16];
     __shared__ Term tm[kMaxTerms];
     __shared__ int off[kKinds + 1];
@@ -388,46 +402,38 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
             }
             __syncthreads();
         }
-        // two rounds: [rotate(k)] [phase block] / [rotate(k+1)]; one copy of the rotation code
-#pragma unroll 1
-        for (int round = 0; round < 2; ++round) {
-        if (round == 0 ? pa.rot_a : pa.rot_b)
-            rotate_group(v, sh, pa.mask, round == 0 ? pa.ca : pa.cb, round == 0 ? pa.sa : pa.sb);
-        if (PH && round == 0 && pa.fast) {
-            // phase block, one sweep. In lay0 amplitude j = t + 256 r: its L bits
-            // (j & 63) are fixed per thread and its H bits are j >> 6, so
-            // E(j) = C + T1[t & 63] + T2[j >> 6] + xE[k(j)] with k(j) the sign
-            // pattern of the L-H couplings; one phasor exp(-i s_c E) applies
-            // phase_out and phase_in together. rho01 pairs (j, j ^ 1) sit in lanes
-            // (t, t ^ 1) and need the phase_out factor Po(j) conj(Po(j ^ 1)), in
-            // which C and T2 cancel: P1o[lo] conj(P1o[lo ^ 1]) xPo[k] conj(xPo[k ^ x0mask]).
-            // The tile goes through each thread's own LDS slots (no barrier: a
-            // thread reads back only what it wrote) so the loop need not be unrolled.
-            const int lo6 = t & 63;
-            const double e_t = cst + tabE[lo6];
-            const double2 pc_t = cmul(cstPo, tabPo[lo6]);
+        if (RA) rotate_group_fwd(v, sh, pa.mask, pa.ca, pa.sa);  // lay0 -> lay1
+        if (PH && pa.fast) {
+            // phase block, one sweep: E(j) = C + T1[j & 63] + T2[j >> 6] + xE[k(j)]
+            // with k(j) the sign pattern of the L-H couplings; one phasor
+            // exp(-i s_c E) applies phase_out and phase_in together. rho01 pairs
+            // (j, j ^ 1) sit in lanes (t, t ^ 1) of the same register slot (bit 0
+            // of j is bit 0 of t in lay0 and lay1) and need the phase_out factor
+            // Po(j) conj(Po(j ^ 1)), in which C and T2 cancel:
+            // P1o[lo] conj(P1o[lo ^ 1]) xPo[k] conj(xPo[k ^ x0mask]). The tile goes
+            // through each thread's own LDS slots (no barrier: a thread reads back
+            // only what it wrote) so the loop need not be unrolled.
             const bool even = (t & 1) == 0;
-            double2 r1 = make_double2(1.0, 0.0);
-            if (pa.phase_out) r1 = cmul(tabPi[lo6], cconj(tabPi[lo6 ^ 1]));
 #pragma unroll
-            for (int r = 0; r < kRegs; ++r) sh[sidx(lay0(t, r))] = v[r];
+            for (int r = 0; r < kRegs; ++r) sh[sidx(lay_of(cur, t, r))] = v[r];
 #pragma unroll 1
             for (int r = 0; r < kRegs; ++r) {
-                const int j = t + (r << 8);
+                const int j = lay_of(cur, t, r);
                 int k = 0;
 #pragma unroll
                 for (int q = 0; q < kFastCross; ++q)
                     if (q < pa.nx) k |= (((j >> pa.xa[q]) ^ (j >> pa.xb[q])) & 1) << q;
-                const int hi = j >> 6;
+                const int lo = j & 63, hi = j >> 6;
                 const double2 u = sh[sidx(j)];
                 if (pa.phase_out) {
                     const double p = u.x * u.x + u.y * u.y;
-                    const double e = e_t + tabE[64 + hi] + xE[k];
+                    const double e = cst + tabE[lo] + tabE[64 + hi] + xE[k];
                     acc[4] += p;
                     acc[5] += p * e;
                     if (pa.best) best = better(best, Best{p, (long long)gidx(j)});
                     const double2 w = make_double2(__shfl_xor(u.x, 1, 64), __shfl_xor(u.y, 1, 64));
                     if (even) {
+                        const double2 r1 = cmul(tabPi[lo], cconj(tabPi[lo ^ 1]));
                         const double2 corr = cmul(r1, cmul(crossPi[k], cconj(crossPi[k ^ pa.x0mask])));
                         const double2 uw = make_double2(u.x * w.x + u.y * w.y, u.y * w.x - u.x * w.y);
                         const double2 z = cmul(uw, corr);
@@ -437,15 +443,16 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
                         acc[3] += z.y;
                     }
                 }
-                sh[sidx(j)] = cmul(u, cmul(pc_t, cmul(tabPo[64 + hi], crossPo[k])));
+                const double2 ph = cmul(cmul(cstPo, tabPo[lo]), cmul(tabPo[64 + hi], crossPo[k]));
+                sh[sidx(j)] = cmul(u, ph);
             }
 #pragma unroll
-            for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay0(t, r))];
+            for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay_of(cur, t, r))];
         }
-        if (PH && round == 0 && !pa.fast) {
+        if (PH && !pa.fast) {
             // phase block on the LDS copy of the tile (lay0 order), plain loops keep registers low
 #pragma unroll
-            for (int r = 0; r < kRegs; ++r) sh[sidx(lay0(t, r))] = v[r];
+            for (int r = 0; r < kRegs; ++r) sh[sidx(lay_of(cur, t, r))] = v[r];
             __syncthreads();
             if (pa.phase_out) {
 #pragma unroll 1
@@ -477,12 +484,17 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
                 __syncthreads();
             }
 #pragma unroll
-            for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay0(t, r))];
+            for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay_of(cur, t, r))];
             __syncthreads();
         }
+        if (RB) {
+            if (cur == kLay1)
+                rotate_group_bwd(v, sh, pa.mask, pa.cb, pa.sb);
+            else
+                rotate_group_fwd(v, sh, pa.mask, pa.cb, pa.sb);
         }
 #pragma unroll
-        for (int r = 0; r < kRegs; ++r) psi[gidx(lay0(t, r))] = v[r];
+        for (int r = 0; r < kRegs; ++r) psi[gidx(lay_of(out_lay, t, r))] = v[r];
         if (PH) __syncthreads();  // tables are rebuilt for the next tile
     }
     if (PH && pa.phase_out) {
@@ -739,12 +751,17 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
         for (int g = 0; g < ng; ++g) pts[g].t = d_terms.p + starts[g];
 
         auto launch = [&](int g, const PassArgs& pa, bool ph) {
-            if (ph)
-                hipLaunchKernelGGL((k_sv_pass<true>), dim3(nblocks), dim3(kThreads), 0, st, psi.p, pts[g], pa, ntiles,
-                                   partials.p, bp.p, bx.p);
-            else
-                hipLaunchKernelGGL((k_sv_pass<false>), dim3(nblocks), dim3(kThreads), 0, st, psi.p, pts[g], pa,
-                                   ntiles, partials.p, bp.p, bx.p);
+            // the passes this driver issues: rotate only; phase then rotate (first);
+            // rotate, phase, rotate (step boundary); rotate then phase (last)
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kThreads), 0, st, psi.p, pts[g], pa, ntiles, partials.p,
+                                   bp.p, bx.p);
+            };
+            if (!ph && pa.rot_a && !pa.rot_b) go(k_sv_pass<false, true, false>);
+            else if (ph && !pa.rot_a && pa.rot_b) go(k_sv_pass<true, false, true>);
+            else if (ph && pa.rot_a && pa.rot_b) go(k_sv_pass<true, true, true>);
+            else if (ph && pa.rot_a && !pa.rot_b) go(k_sv_pass<true, true, false>);
+            else throw Error(-1, "state-vector pass: unsupported rotation pattern");
             TQ_LAUNCH_CHECK();
         };
         auto base_args = [&](int g) {

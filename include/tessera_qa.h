@@ -95,6 +95,15 @@ int tessera_sv_anneal(const double* h, size_t n,
                       int8_t* out_s, double* out_prob, double* out_entropy, double* out_energy_trace,
                       double* out_energy, double* out_expected_energy);
 
+/* tessera_sv_anneal that also returns the final state (n <= 30 here, 2^n
+ * complex amplitudes interleaved re/im in out_state, bit i of the index = spin i
+ * set to +1): the amplitudes upstream quantum.rs:78-87 holds after the last step.
+ * Scalars as tessera_sv_anneal (nullable). */
+int tessera_sv_anneal_state(const double* h, size_t n,
+                            const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                            const double* schedule_a, const double* schedule_b, size_t steps, double dt,
+                            int8_t* out_s, double* out_expected_energy, double* out_entropy, double* out_state);
+
 /*
  * Drop-in for upstream rust/src/capi.rs:107-143 (the ctypes path of
  * python/test_ffi.py and python/tessera_cam.py): state-vector anneal on the

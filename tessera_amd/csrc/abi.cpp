@@ -175,6 +175,45 @@ int tessera_sv_anneal(const double* h, size_t n, const size_t* ei, const size_t*
     }
 }
 
+int tessera_sv_anneal_state(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew,
+                            size_t m, const double* schedule_a, const double* schedule_b, size_t steps, double dt,
+                            int8_t* out_s, double* out_expected_energy, double* out_entropy, double* out_state) {
+    try {
+        set_last_error("");
+        check_model(h, n, ei, ej, ew, m, schedule_a, schedule_b, steps);
+        if (!out_s || !out_state) throw Error(TESSERA_ERR_ARG, "out_s or out_state is null");
+        if (steps < 1) throw Error(TESSERA_ERR_ARG, "steps < 1");
+        if (!std::isfinite(dt)) throw Error(TESSERA_ERR_NONFINITE, "dt is not finite");
+        if (n > 30) throw Error(TESSERA_ERR_TOO_LARGE, "state download limited to n <= 30");
+        SvProblem pb;
+        pb.n = (int)n;
+        pb.h.assign(h, h + n);
+        pb.ei.resize(m);
+        pb.ej.resize(m);
+        for (size_t k = 0; k < m; ++k) {
+            pb.ei[k] = (int)ei[k];
+            pb.ej[k] = (int)ej[k];
+        }
+        pb.ew.assign(ew, ew + m);
+        pb.a.assign(schedule_a, schedule_a + steps);
+        pb.b.assign(schedule_b, schedule_b + steps);
+        pb.steps = (int)steps;
+        pb.dt = dt;
+        const SvOutcome r = sv_anneal_device(pb, true);
+        std::memcpy(out_s, r.spins.data(), n);
+        if (out_expected_energy) *out_expected_energy = r.expected_energy;
+        if (out_entropy) std::memcpy(out_entropy, r.entropy.data(), steps * sizeof(double));
+        std::memcpy(out_state, r.state.data(), r.state.size() * sizeof(double));
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    } catch (...) {
+        set_last_error("unknown exception");
+        return TESSERA_ERR_DEVICE;
+    }
+}
+
 // rust/src/capi.rs:107-143 semantics on the device state vector.
 double tessera_solve_qa(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew, size_t m,
                         size_t steps, double dt, int8_t* out_s, double* out_prob, double* out_entropy,

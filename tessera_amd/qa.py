@@ -56,6 +56,9 @@ def lib() -> ctypes.CDLL:
         L.tessera_sv_anneal.restype = ctypes.c_int
         L.tessera_sv_anneal.argtypes = [vp, _usz, vp, vp, vp, _usz, vp, vp, _usz, ctypes.c_double, vp, vp, vp, vp,
                                         vp, vp]
+        L.tessera_sv_anneal_state.restype = ctypes.c_int
+        L.tessera_sv_anneal_state.argtypes = [vp, _usz, vp, vp, vp, _usz, vp, vp, _usz, ctypes.c_double, vp, vp, vp,
+                                              vp]
         L.tessera_solve_qa.restype = ctypes.c_double
         L.tessera_solve_qa.argtypes = [vp, _usz, vp, vp, vp, _usz, _usz, ctypes.c_double, vp, vp, vp, vp]
         L.tessera_tebd_create.restype = ctypes.c_int
@@ -223,6 +226,22 @@ def anneal_statevector(model: Ising, sched: Schedule, dt: float) -> QaOutcome:
                                    _p(spins), ctypes.addressof(prob), _p(ent), _p(etr), ctypes.addressof(en),
                                    ctypes.addressof(ex)))
     return QaOutcome(spins, en.value, ex.value, prob.value, ent, etr)
+
+
+def statevector_final(model: Ising, sched: Schedule, dt: float):
+    """Final state of anneal_statevector on the device (n <= 30): (psi, <H>, entropy
+    trace), psi[x] with bit i of x set <-> spin i = +1."""
+    ei, ej, ew = _edges(model)
+    n, steps = model.n, sched.steps
+    spins = np.zeros(n, np.int8)
+    ex = ctypes.c_double()
+    ent = np.zeros(steps)
+    state = np.zeros(2 << n)
+    a = np.ascontiguousarray(sched.a, np.float64)
+    b = np.ascontiguousarray(sched.b, np.float64)
+    _check(lib().tessera_sv_anneal_state(_p(model.h), n, _p(ei), _p(ej), _p(ew), model.m, _p(a), _p(b), steps, dt,
+                                         _p(spins), ctypes.addressof(ex), _p(ent), _p(state)))
+    return state[0::2] + 1j * state[1::2], ex.value, ent
 
 
 def solve_qa(h, ei, ej, ew, steps: int, dt: float):

@@ -87,6 +87,23 @@ def test_exact_chi_matches_dense(n, chi, steps):
     assert g["max_trunc"] <= 1e-20
 
 
+def test_exact_chi_long_range_matches_dense():
+    """Random 3-regular graph on 10 spins (edges of range up to 9, routed by long
+    swap chains; mps.cpp:126-145) at chi = 32, the largest bond of 10 sites:
+    nothing is truncated, so the routed evolution equals the dense one."""
+    model = I.random_regular3(10, 31)
+    assert model.max_range() >= 5
+    sched = I.linear(12)
+    g = _gpu_run(model, sched, 0.1, 32)
+    psi, ents = dense_mps_splitting(model, sched, 0.1)
+    inf = 1 - _fid(psi, g["state"])
+    print(f"range {model.max_range()}: infidelity={inf:.2e} ent_err={np.abs(g['entropy'] - ents).max():.2e}",
+          flush=True)
+    assert inf <= 1e-10
+    assert np.abs(g["entropy"] - ents).max() <= 1e-9
+    assert g["max_trunc"] <= 1e-20
+
+
 def test_config1_prefix_matches_accurate_oracle():
     inst = I.config1()
     sched = I.linear(200)

@@ -69,8 +69,9 @@ def parse():
     ap.add_argument("--fractions", default="0.25,0.5", help="learned workload: shortened anneal lengths")
     ap.add_argument("--strengths", default="0.5,1.0", help="learned workload: warm-start bias strengths")
     ap.add_argument("--instances", type=int, default=32, help="batch workload: number of config-4 instances")
-    ap.add_argument("--exec-mode", type=int, default=0, choices=(0, 1, 2),
-                    help="batch workloads: executor layout 0 auto, 1 grouped, 2 solo (one workgroup per update)")
+    ap.add_argument("--exec-mode", type=int, default=0, choices=(-1, 0, 1, 2),
+                    help="batch workloads: executor layout 0 auto, 1 grouped, 2 solo (one workgroup per update), "
+                         "-1 measured by the layout tuner (tessera_amd.tuning)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (PMC byte-counter calibration) and exit")
     return ap.parse_args()
@@ -293,9 +294,16 @@ def batch_bench(args):
         s.close()
     t_solo = sum(solo_times)
     progress(f"one at a time: {len(insts)} instances x {args.steps} steps in {t_solo:.2f}s")
+    mode = args.exec_mode
+    if mode == -1:
+        from tessera_amd import tuning
+
+        rec = tuning.tune(max(x.chi for x in insts), len(insts))
+        mode = rec["mode"]
+        progress(f"layout tuner: {rec}")
     ss = sessions()
     t0 = time.perf_counter()
-    qa.batch_run(ss, [sched(x) for x in insts], args.executor or -1, args.exec_mode)
+    qa.batch_run(ss, [sched(x) for x in insts], args.executor or -1, mode)
     for s in ss:
         s.sync()
     t_batch = time.perf_counter() - t0
@@ -318,7 +326,7 @@ def batch_bench(args):
                    "one_at_a_time_instance_s_p50": float(np.percentile(st, 50)),
                    "one_at_a_time_instance_s_p99": float(np.percentile(st, 99)),
                    "batched_instance_s": t_batch,
-                   "exec_mode": args.exec_mode, "supersteps": ex["supersteps"],
+                   "exec_mode": mode, "supersteps": ex["supersteps"],
                    "updates_per_superstep": ex["updates"] / max(ex["supersteps"], 1)},
     }
     print(json.dumps(out), flush=True)

@@ -19,6 +19,7 @@
 // Rotation cos/sin are computed on the host with the same libm calls as upstream;
 // phases come from per-tile tables (below), so results agree with upstream to
 // rounding (not bitwise).
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -358,6 +359,36 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
         }
         __syncthreads();
     }
+    // Fast path, pass constants kept in registers across tiles: thread t < 128 holds
+    // the base-independent part Tc of table entry t (LL edges and L fields for
+    // t < 64, HH edges and H fields for t >= 64) and its phasor exp(-i s_c Tc);
+    // threads 128..191 hold exp(-i s_out Tc) of the L entries; threads 0..11 hold
+    // the range of base couplings of local bit t (BL / BH terms sorted by local bit).
+    double tc = 0.0;
+    double2 pcc = make_double2(1.0, 0.0);
+    int fb0 = 0, fb1 = 0;
+    if (PH && pa.fast) {
+        if (t < 192) {
+            const int u = t < 128 ? (t & 63) : t - 128;
+            const bool lowh = t < 64 || t >= 128;
+            const int kp = lowh ? kLL : kHH, kf = lowh ? kFL : kFH;
+            for (int q = off[kp]; q < off[kp + 1]; ++q) tc -= tm[q].w * bit_spin(u, tm[q].a) * bit_spin(u, tm[q].b);
+            for (int q = off[kf]; q < off[kf + 1]; ++q) tc -= tm[q].w * bit_spin(u, tm[q].a);
+            double sn, cs;
+            sincos((t < 128 ? s_c : pa.scale_out) * tc, &sn, &cs);
+            pcc = make_double2(cs, -sn);
+        }
+        if (t < 12) {
+            const int kb = t < 6 ? kBL : kBH, bit = t % 6;
+            fb0 = off[kb + 1];
+            fb1 = off[kb + 1];
+            for (int q = off[kb]; q < off[kb + 1]; ++q)
+                if (tm[q].b == bit) {
+                    fb0 = min(fb0, q);
+                    fb1 = q + 1;
+                }
+        }
+    }
     const int c = pa.c, lo = pa.lo;
     const int lo_span = lo - c;  // tile-id bits placed in [c, lo)
     const unsigned long long cmask = (1ull << c) - 1ull;
@@ -380,23 +411,67 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
         double2 v[kRegs];
 #pragma unroll
         for (int r = 0; r < kRegs; ++r) v[r] = psi[gidx(lay0(t, r))];  // loads in flight during the tables
-        if (PH) {
-            build_energy(tm, off, base, tabE, &cst);
+        if (PH && pa.fast) {
+            // The tile's base enters only through the base-local couplings, i.e. a
+            // field f_a = sum_b w s_b(base) on local bit a: T[j] = Tc[j] - sum_a f_a
+            // s_a(j), and exp(-i s T) = exp(-i s Tc) * prod_a exp(+i s f_a s_a), so a
+            // tile needs 19 sincos (fields, constant) instead of one per entry.
+            // Scratch in tabPi[64..127] (unused by the fast path): f at 64..75,
+            // exp(i s_c f) at 76..87, exp(i s_out f) of the L bits at 88..93.
+            double2* fscr = tabPi + 64;
+            if (t < 12) {
+                double f = 0.0;
+                for (int q = fb0; q < fb1; ++q)
+                    if (tm[q].b == t % 6) f += tm[q].w * bit_spin(base, tm[q].a);
+                fscr[t].x = f;
+            } else if (t >= 128 && t < 192) {  // C: BB edges and B fields
+                const int lane = t & 63;
+                double v0 = 0.0;
+                for (int q = off[kBB] + lane; q < off[kBB + 1]; q += 64)
+                    v0 -= tm[q].w * bit_spin(base, tm[q].a) * bit_spin(base, tm[q].b);
+                for (int q = off[kFB] + lane; q < off[kFB + 1]; q += 64) v0 -= tm[q].w * bit_spin(base, tm[q].a);
+                v0 = wave_sum(v0);
+                if (lane == 0) cst = v0;
+            }
             __syncthreads();
-            if (pa.fast) {
-                // tabPo = exp(-i s_c T), tabPi[0..63] = exp(-i s_out T1), cstPo = exp(-i s_c C)
+            {
                 double sn, cs;
-                if (t < 128) {
-                    sincos(s_c * tabE[t], &sn, &cs);
-                    tabPo[t] = make_double2(cs, -sn);
-                } else if (t < 192) {
-                    sincos(pa.scale_out * tabE[t - 128], &sn, &cs);
-                    tabPi[t - 128] = make_double2(cs, -sn);
+                if (t < 12) {
+                    sincos(s_c * fscr[t].x, &sn, &cs);
+                    fscr[12 + t] = make_double2(cs, sn);
+                } else if (t < 18) {
+                    sincos(pa.scale_out * fscr[t - 12].x, &sn, &cs);
+                    fscr[24 + t - 12] = make_double2(cs, sn);
                 } else if (t == 192) {
                     sincos(s_c * cst, &sn, &cs);
                     cstPo = make_double2(cs, -sn);
                 }
-            } else {
+                if (t < 128) {
+                    const int u = t & 63, half = t < 64 ? 0 : 6;
+                    double e = tc;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) e -= fscr[half + a].x * bit_spin(u, a);
+                    tabE[t] = e;
+                }
+            }
+            __syncthreads();
+            if (t < 192) {
+                const int u = t < 128 ? (t & 63) : t - 128;
+                const double2* g = t < 128 ? fscr + 12 + (t < 64 ? 0 : 6) : fscr + 24;
+                double2 ph = pcc;
+#pragma unroll
+                for (int a = 0; a < 6; ++a) ph = cmul(ph, ((u >> a) & 1) ? g[a] : cconj(g[a]));
+                if (t < 128)
+                    tabPo[t] = ph;  // exp(-i s_c T)
+                else
+                    tabPi[t - 128] = ph;  // exp(-i s_out T1)
+            }
+            __syncthreads();
+        }
+        if (PH && !pa.fast) {
+            build_energy(tm, off, base, tabE, &cst);
+            __syncthreads();
+            {
                 if (pa.phase_out) build_phasors(tm, off, pa.scale_out, tabE, cst, tabPo, crossPo, &cstPo);
                 if (pa.phase_in) build_phasors(tm, off, pa.scale_in, tabE, cst, tabPi, crossPi, &cstPi);
             }
@@ -735,6 +810,10 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
             }
             if (bucket[kLH].size() > (std::size_t)kMaxCrossT)
                 throw Error(-1, "too many couplings between the two halves of a tile");
+            // base-local couplings grouped by local bit (the per-bit fields of the tables)
+            for (int kd : {kBL, kBH})
+                std::stable_sort(bucket[kd].begin(), bucket[kd].end(),
+                                 [](const Term& x, const Term& y) { return x.b < y.b; });
             starts[g] = terms.size();
             int acc = 0;
             for (int kd = 0; kd < kKinds; ++kd) {

@@ -108,3 +108,29 @@ def test_batch_config4_sample_runs():
         print(f"rank difference solo - grouped: max {np.abs(b2.astype(int) - b1).max()}", flush=True)
         assert abs(e2 - e1) <= 1e-9
         assert t2 <= 1e-20 and t1 <= 1e-20
+
+
+def test_layout_tuner_picks_a_measured_layout_and_runs_it(tmp_path):
+    """The tuner measures both layouts on saturated probe chains, caches the
+    faster (also in a JSON file) and the tuned batch equals a direct batch in
+    that layout."""
+    from tessera_amd import tuning
+
+    cache = tmp_path / "layouts.json"
+    rec = tuning.tune(16, 6, cache_path=cache)
+    assert rec["mode"] in (1, 2) and set(rec["updates_per_s"]) == {"grouped", "solo"}
+    assert all(v > 0 for v in rec["updates_per_s"].values())
+    assert cache.exists() and tuning.tune(16, 5, cache_path=cache) is rec  # same class, cached
+    cases = _mixed()[:3]
+    out = []
+    for tuned in (True, False):
+        sessions = [qa.TebdSession(model, chi, dt) for model, _, dt, chi in cases]
+        if tuned:
+            tuning.tuned_batch_run(sessions, [c[1] for c in cases], [c[3] for c in cases])
+        else:
+            qa.batch_run(sessions, [c[1] for c in cases], -1, tuning.tune(64, 3)["mode"])
+        out.append([s.dense() for s in sessions])
+        for s in sessions:
+            s.close()
+    for a, b in zip(*out):
+        assert 1 - _fid(a, b) <= 1e-12

@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--epochs", type=int, default=60, help="learned workload: training epochs")
     ap.add_argument("--fractions", default="0.25,0.5", help="learned workload: shortened anneal lengths")
     ap.add_argument("--strengths", default="0.5,1.0", help="learned workload: warm-start bias strengths")
+    ap.add_argument("--exec-workers", type=int, default=0,
+                    help="lockstep executor: workgroups per group (0: one per block pair of a round)")
     ap.add_argument("--instances", type=int, default=32, help="batch workload: number of config-4 instances")
     ap.add_argument("--exec-mode", type=int, default=0, choices=(-1, 0, 1, 2),
                     help="batch workloads: executor layout 0 auto, 1 grouped, 2 solo (one workgroup per update), "
@@ -518,7 +520,7 @@ def main():
     sched = I.linear(SCHED_STEPS)
     k0 = SCHED_STEPS // 2
     sess = qa.TebdSession(model, args.chi, DT)
-    sess.set_executor(args.executor)
+    sess.set_executor(args.executor, args.exec_workers)
     sess.randomize(SEED, 40.0)
     ups = sess.updates_per_step
     a = sched.a[k0:k0 + args.warmup + args.steps]
@@ -579,7 +581,8 @@ def main():
         "config": {"workload": f"MPS-TEBD Trotter step n={args.n} chi={args.chi} range<=4 dt={DT}",
                    "updates_per_step": ups, "max_bond": int(bonds.max()),
                    "svd_calls": calls, "jacobi_sweeps_per_svd": round(sweeps, 2),
-                   "max_trunc": trunc, "executor_groups": args.executor},
+                   "max_trunc": trunc, "executor_groups": args.executor,
+                   "executor_workers": args.exec_workers},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": kernel, **timing},
     }

@@ -180,6 +180,11 @@ int tessera_tebd_set_timing(tessera_tebd* t, int on);
  * per-step entropy is taken from site snapshots. Results equal the launch-per-
  * update path up to rounding. */
 int tessera_tebd_set_executor(tessera_tebd* t, int groups);
+/* tessera_tebd_set_executor with the workgroups per group of the grouped layout:
+ * 0 = one per block pair of a tournament round (the default), w > 0 = at most w,
+ * each taking several pairs per round (fewer workgroups in every barrier). In a
+ * batched run the first session's setting applies. */
+int tessera_tebd_set_executor_shape(tessera_tebd* t, int groups, int workers);
 /* Batched anneal (no upstream counterpart; the batched scheduler): advance
  * count sessions together, session i by steps[i] Trotter steps with schedule
  * values a[i][k], b[i][k], in one cooperative launch whose super-steps take

@@ -446,10 +446,15 @@ int tessera_tebd_set_timing(tessera_tebd* t, int on) {
 }
 
 int tessera_tebd_set_executor(tessera_tebd* t, int groups) {
+    return tessera_tebd_set_executor_shape(t, groups, 0);
+}
+
+int tessera_tebd_set_executor_shape(tessera_tebd* t, int groups, int workers) {
     try {
         if (!t) throw Error(TESSERA_ERR_ARG, "null session");
         if (groups < -1) throw Error(TESSERA_ERR_ARG, "groups must be >= -1");
-        t->eng->set_executor(groups);
+        if (workers < 0) throw Error(TESSERA_ERR_ARG, "workers must be >= 0");
+        t->eng->set_executor(groups, workers);
         return TESSERA_OK;
     } catch (const std::exception& e) {
         set_last_error(e.what());

@@ -88,24 +88,4 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// DPP move of a double within 16-lane rows (CTRL is a DPP control code).
-template <int CTRL>
-__device__ inline double dpp_d(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-// Sum over the 64 lanes, result in every lane: quad_perm and row_ror inside each
-// 16-lane row (DPP, no LDS round trip), then two xor shuffles across rows.
-__device__ inline double wave_sum_dpp(double v) {
-    v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
-    v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
-    v += dpp_d<0x124>(v);  // row_ror 4
-    v += dpp_d<0x128>(v);  // row_ror 8
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    return v;
-}
-
 }  // namespace tq

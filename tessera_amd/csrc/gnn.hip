@@ -32,12 +32,6 @@ __host__ __device__ inline uint16_t to_bf16(float f) {
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
-__host__ __device__ inline float from_bf16(uint16_t b) {
-    const uint32_t u = (uint32_t)b << 16;
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
-}
 
 // C[z] = A B over the K range [z*kchunk, min(K, (z+1)*kchunk)). 256 threads:
 // four wavefronts, each one 16x16 output block of the 32x32 workgroup tile.

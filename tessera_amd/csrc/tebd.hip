@@ -1722,21 +1722,25 @@ void Tebd::run(const double* a, const double* b, int count, int rec0) {
     for (int k = 0; k < count; ++k) step(a[k], b[k], rec0 >= 0 ? rec0 + k : -1);
 }
 
-void Tebd::set_executor(int groups) {
+void Tebd::set_executor(int groups, int workers) {
     if (groups < -1) throw Error(-1, "executor groups must be >= -1");
+    if (workers < 0) throw Error(-1, "executor workers must be >= 0");
     sync();
     exec_req_ = groups;
+    exec_workers_ = workers;
 }
 
 // Groups of `per_group` workgroups (enough for the widest block-Jacobi round of
 // a theta with rows and columns up to 2*maxcap), as many groups as are
 // co-resident (at most `req` if req > 0, else 64), a multiple of 8 when 8 or
 // more fit.
-static void exec_shape(int maxcap, int req, bool solo, int& groups, int& per_group, std::size_t& shm) {
+static void exec_shape(int maxcap, int req, bool solo, int workers, int& groups, int& per_group, std::size_t& shm) {
     const int cmax = 2 * maxcap;
     const int nb = (cmax + kBlk - 1) / kBlk;
     const int nbp = std::max(2, nb + (nb & 1));
-    per_group = solo ? 1 : nbp / 2;
+    // grouped: one worker per block pair of a tournament round by default; fewer
+    // workers take several pairs each (fewer workgroups in every barrier)
+    per_group = solo ? 1 : (workers > 0 ? std::min(workers, nbp / 2) : nbp / 2);
     const std::size_t svd_bytes = (std::size_t)2 * kBlk * (2 * maxcap) * sizeof(double2);
     const std::size_t theta_bytes = (std::size_t)(kDistThreads / 256) * 2 * 16 * 2 * 17 * sizeof(double2);
     // solo: room for a kSmall x kSmall theta of svd_small
@@ -1782,7 +1786,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     const bool solo = mode == 2 || (mode == 0 && ts.size() >= 4 && maxcap <= 32);
     int G = 0, W = 0;
     std::size_t shm = 0;
-    exec_shape(maxcap, req, solo, G, W, shm);
+    exec_shape(maxcap, req, solo, ts[0]->exec_workers_, G, W, shm);
     using cd = std::complex<double>;
     struct M2 {
         cd u[2][2];

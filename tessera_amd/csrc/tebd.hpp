@@ -55,8 +55,9 @@ public:
     // sequence per update, or as one persistent launch when the executor is on.
     void run(const double* a, const double* b, int count, int rec0);
     // Lockstep executor: 0 = off, g > 0 = at most g groups of workgroups,
-    // -1 = as many groups as are co-resident (capped at 64).
-    void set_executor(int groups);
+    // -1 = as many groups as are co-resident (capped at 64). workers: workgroups
+    // per group (0 = one per block pair of a tournament round).
+    void set_executor(int groups, int workers = 0);
     int executor_groups() const { return exec_req_; }
     // Batched lockstep run: instance i advances counts[i] steps with schedule
     // values as[i][k], bs[i][k]; entropy recording (rec0 >= 0) needs one instance.
@@ -141,6 +142,7 @@ private:
 
     // lockstep executor state
     int exec_req_ = 0;
+    int exec_workers_ = 0;
     long long exec_supersteps_ = 0, exec_updates_ = 0;
     DevBuf<unsigned char> ex_tasks_;   // schedule: updates, snapshots, rounds, snapshot offsets
     DevBuf<double2> ex_mat_, ex_work_, ex_snap_, ex_gates_, ex_aux_, ex_taus_;

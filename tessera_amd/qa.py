@@ -82,6 +82,8 @@ def lib() -> ctypes.CDLL:
         L.tessera_tebd_set_timing.argtypes = [vp, ctypes.c_int]
         L.tessera_tebd_set_executor.restype = ctypes.c_int
         L.tessera_tebd_set_executor.argtypes = [vp, ctypes.c_int]
+        L.tessera_tebd_set_executor_shape.restype = ctypes.c_int
+        L.tessera_tebd_set_executor_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.tessera_tebd_theta.restype = ctypes.c_int
         L.tessera_tebd_theta.argtypes = [vp, _usz, vp, _usz, vp, vp]
         L.tessera_tebd_batch_run.restype = ctypes.c_int
@@ -363,10 +365,11 @@ class TebdSession:
     def set_timing(self, on: bool = True):
         _check(lib().tessera_tebd_set_timing(self._h, 1 if on else 0))
 
-    def set_executor(self, groups: int = -1):
+    def set_executor(self, groups: int = -1, workers: int = 0):
         """0: one launch sequence per update; g > 0: one lockstep cooperative launch
-        per run() with at most g concurrent groups of workgroups; -1: as many as fit."""
-        _check(lib().tessera_tebd_set_executor(self._h, int(groups)))
+        per run() with at most g concurrent groups of workgroups; -1: as many as fit.
+        workers > 0 caps the workgroups per group (each takes several block pairs)."""
+        _check(lib().tessera_tebd_set_executor_shape(self._h, int(groups), int(workers)))
 
     def theta(self, p: int) -> np.ndarray:
         """Identity-gate theta of sites (p, p+1): the (2Dl, 2Dr) matrix the next

@@ -20,9 +20,9 @@ from test_gpu_tebd import _fid, dense_mps_splitting
 pytestmark = pytest.mark.gpu
 
 
-def _run(model, sched, dt, chi, groups, ns=1, chunks=1, record=True):
+def _run(model, sched, dt, chi, groups, ns=1, chunks=1, record=True, workers=0):
     s = qa.TebdSession(model, chi, dt, ns)
-    s.set_executor(groups)
+    s.set_executor(groups, workers)
     edges = np.linspace(0, sched.steps, chunks + 1).astype(int)
     for lo, hi in zip(edges[:-1], edges[1:]):
         s.run(sched.a[lo:hi], sched.b[lo:hi], record=record)
@@ -33,15 +33,17 @@ def _run(model, sched, dt, chi, groups, ns=1, chunks=1, record=True):
     return dict(entropy=ent, max_trunc=tr, bonds=bonds, spins=spins, expected_energy=sc[1], state=state)
 
 
-@pytest.mark.parametrize("groups", [1, 8, -1])
-def test_executor_exact_chi_matches_dense(groups):
+@pytest.mark.parametrize("groups,workers", [(1, 0), (8, 0), (-1, 0), (-1, 1), (4, 2)])
+def test_executor_exact_chi_matches_dense(groups, workers):
+    """workers > 0: fewer workgroups per group than block pairs per round, so
+    each takes several pairs (workers = 1: one workgroup runs a whole update)."""
     model = I.chain_nnn(8, 48)
     sched = I.linear(20)
-    g = _run(model, sched, 0.1, 16, groups)
+    g = _run(model, sched, 0.1, 16, groups, workers=workers)
     psi, ents = dense_mps_splitting(model, sched, 0.1)
     inf = 1 - _fid(psi, g["state"])
     err = np.abs(g["entropy"] - ents).max()
-    print(f"groups={groups} infidelity={inf:.2e} ent_err={err:.2e}", flush=True)
+    print(f"groups={groups} workers={workers} infidelity={inf:.2e} ent_err={err:.2e}", flush=True)
     assert inf <= 1e-10
     assert err <= 1e-9
     assert g["max_trunc"] <= 1e-20
@@ -127,13 +129,15 @@ def test_executor_n128_matches_stream():
     sched = I.linear(1000)
     a, b = sched.a[500:502], sched.b[500:502]
     out = []
-    for groups in (0, -1):
+    for groups, workers in ((0, 0), (-1, 0), (-1, 8)):
         s = qa.TebdSession(inst, 256, 0.04)
-        s.set_executor(groups)
+        s.set_executor(groups, workers)
         s.run(a, b)
         out.append(s.sync())
         s.close()
-    (e0, t0, b0), (e1, t1, b1) = out
-    print(f"n128 chi256 ent {e0} {e1} trunc {t0:.3e} {t1:.3e} max bond {b0.max()} {b1.max()}", flush=True)
-    assert t0 <= 1e-20 and t1 <= 1e-20
+    (e0, t0, b0), (e1, t1, b1), (e2, t2, b2) = out
+    print(f"n128 chi256 ent {e0} {e1} {e2} trunc {t0:.3e} {t1:.3e} {t2:.3e} max bond {b0.max()} {b1.max()}",
+          flush=True)
+    assert t0 <= 1e-20 and t1 <= 1e-20 and t2 <= 1e-20
     assert np.abs(e0 - e1).max() <= 1e-9
+    assert np.abs(e0 - e2).max() <= 1e-9

@@ -57,11 +57,11 @@ def parse():
     ap.add_argument("--chi", type=int, default=CHI)
     ap.add_argument("--executor", type=int, default=-1,
                     help="lockstep executor groups (-1: as many as fit, 0: one launch sequence per update)")
-    ap.add_argument("--workload", choices=("tebd", "sv", "batch", "learned", "gnn", "config1"), default="tebd",
+    ap.add_argument("--workload", choices=("tebd", "sv", "batch", "learned", "gnn", "chain12"), default="tebd",
                     help="tebd: the headline metric; sv: the n=30 state-vector sweep (HBM roofline); "
-                         "batch: batched anneals of config-4 instances; learned: guidance network "
+                         "batch: batched anneals of the batched-workload instances; learned: guidance network "
                          "(GPU TEBD targets, GPU training, guided vs plain anneals); gnn: training "
-                         "throughput of the guidance network alone; config1: the n=12 CPU-runnable case, "
+                         "throughput of the guidance network alone; chain12: the n=12 chain case, "
                          "MPS-TEBD at chi=64 vs the exact state vector")
     ap.add_argument("--train", type=int, default=512, help="learned workload: training instances")
     ap.add_argument("--eval", type=int, default=32, help="learned workload: held-out instances")
@@ -70,7 +70,7 @@ def parse():
     ap.add_argument("--strengths", default="0.5,1.0", help="learned workload: warm-start bias strengths")
     ap.add_argument("--exec-workers", type=int, default=0,
                     help="lockstep executor: workgroups per group (0: one per block pair of a round)")
-    ap.add_argument("--instances", type=int, default=32, help="batch workload: number of config-4 instances")
+    ap.add_argument("--instances", type=int, default=32, help="batch workload: number of batched-workload instances")
     ap.add_argument("--exec-mode", type=int, default=0, choices=(-1, 0, 1, 2),
                     help="batch workloads: executor layout 0 auto, 1 grouped, 2 solo (one workgroup per update), "
                          "-1 measured by the layout tuner (tessera_amd.tuning)")
@@ -126,7 +126,7 @@ def sv_bench(args):
     from tessera_amd import instances as I
     from tessera_amd import qa
 
-    inst = I.config2(SV_SEED, SV_STEPS)
+    inst = I.regular3_n30_anneal(SV_SEED, SV_STEPS)
     n = inst.model.n
     w, k = max(args.warmup, 1), args.steps
     full = inst.sched
@@ -202,7 +202,7 @@ def config1_bench(args):
     from tessera_amd import instances as I
     from tessera_amd import qa
 
-    inst = I.config1()
+    inst = I.chain12_anneal()
     psi_sv, ex_sv, ent_sv = qa.statevector_final(inst.model, inst.sched, inst.dt)
     g_sv = qa.anneal_statevector(inst.model, inst.sched, inst.dt)
     walls = []
@@ -219,9 +219,9 @@ def config1_bench(args):
     # MPS physical index 0 <-> spin +1, state-vector bit set <-> spin +1
     psi_m = psi_mps[::-1]
     fid = abs(np.vdot(psi_m, psi_sv)) ** 2 / (np.vdot(psi_m, psi_m).real * np.vdot(psi_sv, psi_sv).real)
-    progress(f"config1: fidelity {fid:.9f}, TEBD anneal {wall:.3f}s")
+    progress(f"n=12 chain: fidelity {fid:.9f}, TEBD anneal {wall:.3f}s")
     out = {
-        "metric": "config-1 MPS-TEBD anneals/s (n=12 chain, chi=64, 200 steps) with fidelity to the exact state vector",
+        "metric": "n=12 chain MPS-TEBD anneals/s (n=12 chain, chi=64, 200 steps) with fidelity to the exact state vector",
         "value": 1.0 / wall, "unit": "anneals/s", "n_gpus": 1, "steps": max(args.steps, 1), "warmup": 0,
         "ms_per_step": wall / inst.sched.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "c128 (fp64)", "data": "synthetic: seeded n=12 chain (nn + nnn couplings)",
@@ -269,7 +269,7 @@ def batch_bench(args):
     from tessera_amd import instances as I
     from tessera_amd import qa
 
-    insts = I.config4(args.instances)
+    insts = I.batch_instances(args.instances)
 
     def sessions():
         out = []
@@ -315,10 +315,10 @@ def batch_bench(args):
     progress(f"batched: {t_batch:.2f}s")
     st = np.array(solo_times)
     out = {
-        "metric": "batched TEBD instances/s (config-4 sample, saturated, mid-anneal steps)",
+        "metric": "batched TEBD instances/s (batched-workload sample, saturated, mid-anneal steps)",
         "value": len(insts) / t_batch, "unit": "instances/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
         "ms_per_step": t_batch / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c128 (fp64)", "data": "synthetic: config-4 generator (seeds 4000+i), saturated states",
+        "dtype": "c128 (fp64)", "data": "synthetic: batched-workload generator (seeds 4000+i), saturated states",
         "config": {"workload": f"{len(insts)} instances x {args.steps} Trotter steps, chi in {{32,64,128}}",
                    "two_site_updates": total_updates,
                    "batched_updates_per_s": total_updates / t_batch,
@@ -335,7 +335,7 @@ def batch_bench(args):
 
 
 def learned_bench(args):
-    """Learned guidance, a reduced config-5 sample: training instances generated
+    """Learned guidance, a reduced sample of the learned-guidance workload: training instances generated
     like the batched workload (n ~ U{16..40}, chi = 32, T = 10 at dt = 0.05),
     targets from batched GPU TEBD anneals (final <Z_i>, entanglement-peak
     fraction), the network trained on the GPU, then held-out instances (n ~
@@ -411,7 +411,7 @@ def learned_bench(args):
     gnn_ref.loss_and_grad(w, sub)
     t_cpu = time.perf_counter() - tc
     out = {
-        "metric": "time-to-1% energy, learned vs plain (guided wall / plain wall; reduced config-5 sample)",
+        "metric": "time-to-1% energy, learned vs plain (guided wall / plain wall; reduced learned-guidance sample)",
         "value": pick["wall_ratio"] if pick else None, "unit": "ratio", "n_gpus": 1, "steps": steps,
         "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16 MFMA / fp32 accumulate (network); c128 (TEBD)",
@@ -490,7 +490,7 @@ def gnn_bench(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding is a design in DESIGN.md",
+        print(json.dumps({"error": "this implementation runs on one GPU; multi-GPU sharding is designed, not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.copy_calibration:
@@ -508,7 +508,7 @@ def main():
     if args.workload == "gnn":
         gnn_bench(args)
         return
-    if args.workload == "config1":
+    if args.workload == "chain12":
         config1_bench(args)
         return
     import numpy as np

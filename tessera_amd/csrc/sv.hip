@@ -830,7 +830,7 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
         for (int g = 0; g < ng; ++g) pts[g].t = d_terms.p + starts[g];
 
         auto launch = [&](int g, const PassArgs& pa, bool ph) {
-            // the passes this driver issues: rotate only; phase then rotate (first);
+            // the passes the host loop below issues: rotate only; phase then rotate (first);
             // rotate, phase, rotate (step boundary); rotate then phase (last)
             auto go = [&](auto kern) {
                 hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kThreads), 0, st, psi.p, pts[g], pa, ntiles, partials.p,

@@ -244,22 +244,22 @@ def sparse_banded(n: int, rmax: int, mean_degree: float, seed: int, rng: Rng | N
     return make_ising(h, edges)
 
 
-def config1(seed: int = 1) -> Instance:
+def chain12_anneal(seed: int = 1) -> Instance:
     """n=12 chain (nn + nnn), linear schedule, T=10, dt=0.05 (200 steps), chi=64 (exact)."""
     return Instance(chain_nnn(12, seed), linear(200), 0.05, chi=64, name="tfim_n12_chain_nnn")
 
 
-def config2(seed: int = 2, steps: int = 1000) -> Instance:
+def regular3_n30_anneal(seed: int = 2, steps: int = 1000) -> Instance:
     """n=30 random 3-regular, linear schedule, T=20, dt=0.02 (1000 steps)."""
     return Instance(random_regular3(30, seed), linear(steps), 0.02, chi=0, name="sv_n30_3regular")
 
 
-def config3(seed: int = 3, steps: int = 1000, n: int = 128, chi: int = 256) -> Instance:
+def chain128_range4_anneal(seed: int = 3, steps: int = 1000, n: int = 128, chi: int = 256) -> Instance:
     """n=128 chain with couplings of range <= 4, linear schedule, T=40, dt=0.04, chi=256."""
     return Instance(chain_range(n, 4, seed), linear(steps), 0.04, chi=chi, name=f"tebd_n{n}_range4_chi{chi}")
 
 
-def config4(count: int = 512, base_seed: int = 4000) -> list[Instance]:
+def batch_instances(count: int = 512, base_seed: int = 4000) -> list[Instance]:
     """Batched instances: n ~ U{16..96}, chi in {32,64,128}, banded sparse graphs of mean
     degree 3 and range <= 3, own T ~ U[10,40] at dt = 0.05."""
     out = []

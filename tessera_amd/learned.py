@@ -14,7 +14,7 @@ predicts
   schedule stretch: an instance whose predicted peak comes later than the
   training mean gets proportionally more steps.
 
-Plain TEBD stays the fallback and the judge: bias None and stretch 1 give the
+Plain TEBD stays the fallback and the reference point: bias None and stretch 1 give the
 plain anneal exactly, and the evaluation compares guided energies against the
 best plain energy. The structural template is python/gnn_guide.py:81-103
 (IsingGNN); targets, heads and the guidance rule are this library's.

@@ -79,7 +79,7 @@ def main():
     cases.append(case("chain_nnn_n6_chi8", I.chain_nnn(6, 1), ("linear", 30), 0.1, 8))
     cases.append(case("chain_nnn_n8_chi4_trunc", I.chain_nnn(8, 2), ("linear", 20), 0.1, 4))
     cases.append(case("range3_n7_chi4_zz2", I.chain_range(7, 3, 5), ("linear", 12), 0.08, 4, ns=2))
-    cases.append(case("config1_prefix8", I.config1().model, ("linear", 8), 0.05, 64))
+    cases.append(case("chain12_prefix8", I.chain12_anneal().model, ("linear", 8), 0.05, 64))
     for t in range(6):  # cuda/tests/test_mps.cpp:96-123 instances
         model = pyoracle.upstream_instance(123, t, 6)
         cases.append(case(f"upstream_seed123_{t}", model, ("smooth", 1200), 0.08, 8, trace=False))

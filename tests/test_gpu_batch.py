@@ -79,7 +79,7 @@ def test_batch_continues_and_groups_agree(mode):
 
 
 def test_batch_config4_sample_runs():
-    """Eight config-4 instances (n up to 96, chi in {32, 64, 128}) for three
+    """Eight batched-workload instances (n up to 96, chi in {32, 64, 128}) for three
     steps: the batched run completes and every session reads out finite values
     with bonds within chi; the solo and grouped layouts agree on the expected
     energy and discard nothing. Their ranks may differ: three steps from the
@@ -88,7 +88,7 @@ def test_batch_config4_sample_runs():
     small thetas, LQ-preconditioned block Jacobi) round them to different sides
     of the S < 1e-14 keep cut (mps.cpp:77-85); what either drops weighs below
     1e-20. (On the mixed exact cases above the ranks are identical.)"""
-    insts = I.config4(8)
+    insts = I.batch_instances(8)
     scheds = [I.Schedule(x.sched.a[:3], x.sched.b[:3]) for x in insts]
     res = []
     for mode in (2, 1):

@@ -89,13 +89,13 @@ def test_executor_truncated_matches_accurate_oracle():
 
 
 def test_executor_config1_prefix_matches_accurate_oracle():
-    inst = I.config1()
+    inst = I.chain12_anneal()
     sched = I.linear(200)
     sched = I.Schedule(sched.a[:12], sched.b[:12])
     g = _run(inst.model, sched, inst.dt, inst.chi, -1)
     o = pyoracle.tebd(inst.model, sched, inst.dt, inst.chi, svd_mode=1, want_state=True)
     inf = 1 - _fid(o["state"], g["state"])
-    print(f"executor config1[:12] infidelity={inf:.2e}", flush=True)
+    print(f"executor n=12 chain[:12] infidelity={inf:.2e}", flush=True)
     assert inf <= 1e-10
     assert np.abs(g["entropy"] - o["entropy"]).max() <= 1e-9
     assert abs(g["expected_energy"] - o["expected_energy"]) <= 1e-9

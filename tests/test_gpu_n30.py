@@ -33,7 +33,7 @@ pytestmark = pytest.mark.gpu
 def test_tebd_n30_matches_state_vector_prefix():
     import time
 
-    inst = I.config2()
+    inst = I.regular3_n30_anneal()
     steps = 4
     sched = I.Schedule(inst.sched.a[:steps], inst.sched.b[:steps])
     g = qa.anneal_statevector(inst.model, sched, inst.dt)

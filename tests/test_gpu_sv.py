@@ -44,7 +44,7 @@ def _compare(model, sched, dt, tol_trace=1e-10):
 
 
 def test_config1_resident_matches_oracle():
-    inst = I.config1()
+    inst = I.chain12_anneal()
     _compare(inst.model, inst.sched, inst.dt)
 
 

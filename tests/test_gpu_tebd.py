@@ -105,13 +105,13 @@ def test_exact_chi_long_range_matches_dense():
 
 
 def test_config1_prefix_matches_accurate_oracle():
-    inst = I.config1()
+    inst = I.chain12_anneal()
     sched = I.linear(200)
     sched = I.Schedule(sched.a[:12], sched.b[:12])  # first 12 steps of the T=10 run
     g = _gpu_run(inst.model, sched, inst.dt, inst.chi)
     o = pyoracle.tebd(inst.model, sched, inst.dt, inst.chi, svd_mode=1, want_state=True)
     inf = 1 - _fid(o["state"], g["state"])
-    print(f"config1[:12] infidelity={inf:.2e} ent_err={np.abs(g['entropy'] - o['entropy']).max():.2e}", flush=True)
+    print(f"n=12 chain[:12] infidelity={inf:.2e} ent_err={np.abs(g['entropy'] - o['entropy']).max():.2e}", flush=True)
     assert inf <= 1e-10
     assert np.abs(g["entropy"] - o["entropy"]).max() <= 1e-9
     assert abs(g["expected_energy"] - o["expected_energy"]) <= 1e-9

@@ -20,16 +20,16 @@ def test_rng_is_deterministic_and_uniform():
 
 
 def test_config_shapes():
-    c1 = I.config1()
+    c1 = I.chain12_anneal()
     assert c1.model.n == 12 and c1.model.m == 21 and c1.sched.steps == 200 and routed_updates(c1.model) == 41
-    c2 = I.config2(steps=3)
+    c2 = I.regular3_n30_anneal(steps=3)
     deg = np.bincount(np.concatenate([c2.model.ei, c2.model.ej]).astype(int), minlength=30)
     assert c2.model.n == 30 and c2.model.m == 45 and np.all(deg == 3)
-    c3 = I.config3(steps=3)
+    c3 = I.chain128_range4_anneal(steps=3)
     assert c3.model.n == 128 and c3.model.m == 502 and routed_updates(c3.model) == 1998
     assert c3.model.max_range() == 4
     assert np.all(np.abs(c3.model.ew * (c3.model.ej - c3.model.ei).astype(float)) <= 1.0)
-    c4 = I.config4(count=20)
+    c4 = I.batch_instances(count=20)
     assert all(16 <= x.model.n <= 96 and x.chi in (32, 64, 128) and x.model.max_range() <= 3 for x in c4)
 
 

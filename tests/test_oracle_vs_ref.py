@@ -69,7 +69,7 @@ def test_upstream_norm_drift_is_documented():
     normalises their vectors by them (linalg.cpp:128-139, mps.cpp:82); once
     bonds fill, <psi|psi> leaves 1 (fixture from the upstream build). The same
     algorithm with an accurate SVD (svd_mode 1) keeps the norm."""
-    case = [c for c in CASES if c["name"] == "config1_prefix8"][0]
+    case = [c for c in CASES if c["name"] == "chain12_prefix8"][0]
     assert max(abs(x - 1.0) for x in case["norm2"]) > 1e-3
     o = pyoracle.tebd(_model(case), _sched(case), case["dt"], case["chi"], svd_mode=1, want_norms=True)
     assert np.abs(o["norm2"] - 1.0).max() < 1e-10

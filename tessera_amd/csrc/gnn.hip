@@ -335,6 +335,8 @@ void Gnn::set_data(int graphs, const int32_t* node_off, const float* feat, const
     for (int g = 0; g < graphs; ++g)
         if (node_off[g + 1] <= node_off[g]) throw Error(-1, "every graph needs at least one node");
     const int nodes = node_off[graphs];
+    // per-node kernels index nodes * 192 elements in 32-bit arithmetic
+    if (nodes > (1 << 24)) throw Error(-1, "more than 2^24 nodes in one dataset");
     if (adj_off[0] != 0) throw Error(-1, "adj_off[0] must be 0");
     for (int i = 0; i < nodes; ++i)
         if (adj_off[i + 1] < adj_off[i]) throw Error(-1, "adj_off must be non-decreasing");
@@ -348,6 +350,14 @@ void Gnn::set_data(int graphs, const int32_t* node_off, const float* feat, const
         if (!std::isfinite(feat[i])) throw Error(-4, "non-finite node feature");
     for (int e = 0; e < nnz; ++e)
         if (!std::isfinite(adj_j[e])) throw Error(-4, "non-finite coupling");
+    const bool targets = z_target && s_target;
+    if (targets) {
+        for (int i = 0; i < nodes; ++i)
+            if (!std::isfinite(z_target[i])) throw Error(-4, "non-finite target");
+        for (int g = 0; g < graphs; ++g)
+            if (!std::isfinite(s_target[g])) throw Error(-4, "non-finite target");
+    }
+    // everything is checked before the previous dataset is replaced
     TQ_HIP(hipStreamSynchronize(stream_));
     graphs_ = graphs;
     nodes_ = nodes;
@@ -365,12 +375,8 @@ void Gnn::set_data(int graphs, const int32_t* node_off, const float* feat, const
         TQ_HIP(hipMemcpy(adj_j_.p, adj_j, nnz * sizeof(float), hipMemcpyHostToDevice));
     }
     TQ_HIP(hipMemcpy(feat_.p, feat, 2 * (std::size_t)nodes * sizeof(float), hipMemcpyHostToDevice));
-    has_targets_ = z_target && s_target;
+    has_targets_ = targets;
     if (has_targets_) {
-        for (int i = 0; i < nodes; ++i)
-            if (!std::isfinite(z_target[i])) throw Error(-4, "non-finite target");
-        for (int g = 0; g < graphs; ++g)
-            if (!std::isfinite(s_target[g])) throw Error(-4, "non-finite target");
         tz_.alloc(nodes);
         ts_.alloc(graphs);
         TQ_HIP(hipMemcpy(tz_.p, z_target, nodes * sizeof(float), hipMemcpyHostToDevice));

@@ -117,3 +117,36 @@ def test_session_entries_reject_null_sessions():
     assert L.tessera_tebd_batch_run(ctypes.addressof(handles), 0, None, None, None, -2) == -1
     assert L.tessera_tebd_batch_run(ctypes.addressof(handles), 0, None, None, None, -1) == 0  # nothing to do
     assert qa.lib().tessera_tebd_updates_per_step(None) == 0
+
+
+def test_guidance_and_hook_entries_reject_bad_arguments():
+    """The learned-guidance, warm-start and executor-shape entries validate their
+    arguments before touching a device."""
+    from tessera_amd import learned
+
+    L = learned._bind()
+    assert L.tessera_gnn_create(1, None) == -1
+    assert L.tessera_gnn_param_count() == 50050
+    out = np.zeros(4, np.float32)
+    assert L.tessera_gnn_get_params(None, qa._p(out)) == -1
+    assert L.tessera_gnn_set_params(None, qa._p(out)) == -1
+    assert L.tessera_gnn_predict(None, 0, 1, None, None) == -1
+    assert L.tessera_gnn_grad(None, 0, 1, None, None) == -1
+    assert L.tessera_gnn_train_step(None, 0, 1, ctypes.c_float(1e-3), None) == -1
+    off = np.zeros(2, np.int32)
+    assert L.tessera_gnn_set_data(None, 1, qa._p(off), qa._p(out), qa._p(off), None, None, None, None) == -1
+    L.tessera_gnn_destroy(None)
+    Lq = qa.lib()
+    z = np.zeros(4)
+    assert Lq.tessera_tebd_expect_z(None, qa._p(z)) == -1
+    assert Lq.tessera_tebd_entropy_now(None, qa._p(z)) == -1
+    assert Lq.tessera_tebd_set_bias(None, None) == -1
+    assert Lq.tessera_tebd_reset_warm(None, None) == -1
+    assert Lq.tessera_tebd_set_executor_shape(None, -1, 0) == -1
+    handles = (ctypes.c_void_p * 1)(None)
+    assert Lq.tessera_tebd_batch_run_ex(ctypes.addressof(handles), 0, None, None, None, -1, 3) == -1
+    h = np.zeros(2)
+    spins = np.zeros(2, np.int8)
+    st = np.zeros(8)
+    assert Lq.tessera_sv_anneal_state(qa._p(h), 2, None, None, None, 0, None, None, 0, 0.1, qa._p(spins), None, None,
+                                      qa._p(st)) == -1  # steps < 1

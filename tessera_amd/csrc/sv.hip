@@ -322,10 +322,12 @@ __device__ inline void rotate_group_bwd(double2 (&v)[kRegs], double2* sh, unsign
 // PH: phase block (merged pass); RA / RB: rotate before / after it. The tile is
 // loaded in lay0; a rotation from lay0 ends in lay1 and one from lay1 in lay0,
 // so the phase layout and the store layout follow from RA and RB at compile time.
-template <bool PH, bool RA, bool RB>
+template <bool PH, bool RA, bool RB, bool FAST>
 __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ psi, PassTerms pt, PassArgs pa,
                                                       long long ntiles, double* __restrict__ partials,
                                                       double* __restrict__ best_p, long long* __restrict__ best_x) {
+    // FAST: the phase block of the register-slot loop (pa.fast set by the host),
+    // else the LDS phase block with a per-coupling loop; one of them per instance
     constexpr int cur = RA ? kLay1 : kLay0;                         // layout of the phase block
     constexpr int out_lay = RB ? (cur == kLay1 ? kLay0 : kLay1) : cur;  // layout of the store
     __shared__ double2 sh[kTile + kTile / 16];
@@ -345,7 +347,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
     if (PH) {
         for (int q = t; q < pt.off[kKinds]; q += kThreads) tm[q] = pt.t[q];
         if (t <= kKinds) off[t] = pt.off[t];
-        if (pa.fast && t < (1 << pa.nx)) {
+        if (FAST && t < (1 << pa.nx)) {
             // pass constants: energy and phasors of the L-H couplings per sign pattern
             // (bit q of t set <-> the spins of coupling q differ: -w s s = +w)
             double e = 0.0;
@@ -367,7 +369,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
     double tc = 0.0;
     double2 pcc = make_double2(1.0, 0.0);
     int fb0 = 0, fb1 = 0;
-    if (PH && pa.fast) {
+    if (PH && FAST) {
         if (t < 192) {
             const int u = t < 128 ? (t & 63) : t - 128;
             const bool lowh = t < 64 || t >= 128;
@@ -411,7 +413,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
         double2 v[kRegs];
 #pragma unroll
         for (int r = 0; r < kRegs; ++r) v[r] = psi[gidx(lay0(t, r))];  // loads in flight during the tables
-        if (PH && pa.fast) {
+        if (PH && FAST) {
             // The tile's base enters only through the base-local couplings, i.e. a
             // field f_a = sum_b w s_b(base) on local bit a: T[j] = Tc[j] - sum_a f_a
             // s_a(j), and exp(-i s T) = exp(-i s Tc) * prod_a exp(+i s f_a s_a), so a
@@ -468,7 +470,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
             }
             __syncthreads();
         }
-        if (PH && !pa.fast) {
+        if (PH && !FAST) {
             build_energy(tm, off, base, tabE, &cst);
             __syncthreads();
             {
@@ -478,7 +480,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
             __syncthreads();
         }
         if (RA) rotate_group_fwd(v, sh, pa.mask, pa.ca, pa.sa);  // lay0 -> lay1
-        if (PH && pa.fast) {
+        if (PH && FAST) {
             // phase block, one sweep: E(j) = C + T1[j & 63] + T2[j >> 6] + xE[k(j)]
             // with k(j) the sign pattern of the L-H couplings; one phasor
             // exp(-i s_c E) applies phase_out and phase_in together. rho01 pairs
@@ -524,7 +526,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
 #pragma unroll
             for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay_of(cur, t, r))];
         }
-        if (PH && !pa.fast) {
+        if (PH && !FAST) {
             // phase block on the LDS copy of the tile (lay0 order), plain loops keep registers low
 #pragma unroll
             for (int r = 0; r < kRegs; ++r) sh[sidx(lay_of(cur, t, r))] = v[r];
@@ -836,10 +838,11 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
                 hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kThreads), 0, st, psi.p, pts[g], pa, ntiles, partials.p,
                                    bp.p, bx.p);
             };
-            if (!ph && pa.rot_a && !pa.rot_b) go(k_sv_pass<false, true, false>);
-            else if (ph && !pa.rot_a && pa.rot_b) go(k_sv_pass<true, false, true>);
-            else if (ph && pa.rot_a && pa.rot_b) go(k_sv_pass<true, true, true>);
-            else if (ph && pa.rot_a && !pa.rot_b) go(k_sv_pass<true, true, false>);
+            const bool f = pa.fast != 0;
+            if (!ph && pa.rot_a && !pa.rot_b) go(k_sv_pass<false, true, false, false>);
+            else if (ph && !pa.rot_a && pa.rot_b) f ? go(k_sv_pass<true, false, true, true>) : go(k_sv_pass<true, false, true, false>);
+            else if (ph && pa.rot_a && pa.rot_b) f ? go(k_sv_pass<true, true, true, true>) : go(k_sv_pass<true, true, true, false>);
+            else if (ph && pa.rot_a && !pa.rot_b) f ? go(k_sv_pass<true, true, false, true>) : go(k_sv_pass<true, true, false, false>);
             else throw Error(-1, "state-vector pass: unsupported rotation pattern");
             TQ_LAUNCH_CHECK();
         };

@@ -749,14 +749,16 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
 }
 
 // ---- small SVD held in LDS (one workgroup per update) ----------------------
-// For theta with c <= kSmall columns of m <= kSmall rows (bond dimensions up to
-// 32, or larger bonds not yet grown): the whole working copy sits in LDS and
-// every 16-lane group rotates one column pair per tournament round, so a round
-// costs one workgroup barrier instead of the block pipeline of svd_blocks.
-// Rotation, convergence test, noise floor, ordering and keep rule are those of
-// pair_rotate and svd_blocks (mps.cpp:77-85 for the keep rule).
+// For theta with c <= kSmall columns of m <= 16 * RPL rows (bond dimensions up
+// to 32, or larger bonds not yet grown; RPL rows per lane): the whole working
+// copy sits in LDS and every 16-lane group rotates one column pair per
+// tournament round, so a round costs one workgroup barrier instead of the block
+// pipeline of svd_blocks. Rotation, convergence test, noise floor, ordering and
+// keep rule are those of pair_rotate and svd_blocks (mps.cpp:77-85 for the keep
+// rule).
 constexpr int kSmall = 64;
-constexpr int kSmallRows = kSmall / 16;  // rows per lane of a 16-lane group
+constexpr int kSmallRows = kSmall / 16;  // rows per lane of a 16-lane group, m <= 64
+constexpr int kSmallTall = 128;          // rows of the tall form (c <= 64, m <= 128)
 
 __device__ inline double sum16(double v) {  // over the aligned 16-lane group
     v += __shfl_xor(v, 8, 64);
@@ -766,6 +768,7 @@ __device__ inline double sum16(double v) {  // over the aligned 16-lane group
     return v;
 }
 
+template <int RPL>
 __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, int chi, int m, int c,
                           double* __restrict__ sv_out, int* __restrict__ perm_out, double* trunc,
                           unsigned long long* stats, double2* A, SvdShared& sh) {
@@ -819,10 +822,10 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
                 if (i >= c || j >= c) continue;
                 double2* px = A + (std::size_t)i * m;
                 double2* py = A + (std::size_t)j * m;
-                double2 x[kSmallRows], y[kSmallRows];
+                double2 x[RPL], y[RPL];
                 double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
 #pragma unroll
-                for (int q = 0; q < kSmallRows; ++q) {
+                for (int q = 0; q < RPL; ++q) {
                     const int row = l16 + 16 * q;
                     x[q] = row < m ? px[row] : make_double2(0, 0);
                     y[q] = row < m ? py[row] : make_double2(0, 0);
@@ -845,7 +848,7 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
                 const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                 const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
 #pragma unroll
-                for (int q = 0; q < kSmallRows; ++q) {
+                for (int q = 0; q < RPL; ++q) {
                     const int row = l16 + 16 * q;
                     const double2 u = x[q], v = cmul(y[q], ph);
                     if (row < m) {
@@ -914,6 +917,23 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
         sv_out[q] = sh.nrm[sh.perm[q]];
         perm_out[q] = sh.perm[q];
     }
+}
+
+// One update's SVD of a theta with at most kSmall columns and kSmallTall rows
+// (the per-update launch sequence): svd_small on one workgroup of kDistThreads
+// threads, so every column pair of a tournament round has its own 16-lane group.
+__global__ void __launch_bounds__(kDistThreads) k_svd_small(double2* __restrict__ Wg, int ld, int* dims, int p,
+                                                            int chi, double* __restrict__ sv_out,
+                                                            int* __restrict__ perm_out, double* trunc,
+                                                            unsigned long long* stats) {
+    extern __shared__ double2 lds[];
+    __shared__ SvdShared sh;
+    const int R = 2 * dims[p], C = 2 * dims[p + 2];
+    const int m = C <= R ? R : C, c = C <= R ? C : R;
+    if (m <= kSmall)
+        svd_small<kSmallRows>(Wg, ld, dims, p, chi, m, c, sv_out, perm_out, trunc, stats, lds, sh);
+    else
+        svd_small<kSmallTall / 16>(Wg, ld, dims, p, chi, m, c, sv_out, perm_out, trunc, stats, lds, sh);
 }
 
 struct GridSync {
@@ -1411,7 +1431,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 const int R = 2 * I.dims[u.site], C = 2 * I.dims[u.site + 2];
                 const int m = C <= R ? R : C, c = C <= R ? C : R;
                 if (a.small && m <= kSmall && c <= kSmall) {
-                    svd_small(W, a.ld, I.dims, u.site, I.chi, m, c, sv, perm, I.trunc, I.stats, lds, sh);
+                    svd_small<kSmallRows>(W, a.ld, I.dims, u.site, I.chi, m, c, sv, perm, I.trunc, I.stats, lds, sh);
                 } else {
                     SvdMode mode;  // own tournament length, own convergence
                     svd_blocks(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, 0, 1, bs, lds,
@@ -1521,8 +1541,12 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd_dist, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     TQ_HIP(hipFuncSetAttribute((const void*)k_tebd_lockstep, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
+    TQ_HIP(hipFuncSetAttribute((const void*)k_svd_small, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               kSmall * kSmallTall * (int)sizeof(double2)));
     const char* fd = std::getenv("TESSERA_FORCE_DIST_SVD");  // test knob: distributed SVD at every size
     force_dist_ = fd && fd[0] == '1';
+    const char* ss = std::getenv("TESSERA_SMALL_SVD");  // test knob: 0 keeps small thetas on k_svd
+    small_svd_ = !(ss && ss[0] == '0');
     const char* xp = std::getenv("TESSERA_SVD_XCD_PACK");  // 1: keep the SVD's workgroups on one XCD
     xcd_stride_ = (xp && xp[0] == '1') ? 8 : 1;
     build_plan();
@@ -1630,12 +1654,17 @@ void Tebd::two_site(int p, const Gate4& g, const double* wts) {
         }
         TQ_HIP(hipEventRecord(events_[ev_next_].first, stream_));
     }
-    if (bytes <= 144 * 1024 && !force_dist_) {
+    const int ccap = std::min(2 * Dlc, 2 * Drc), mcap = std::max(2 * Dlc, 2 * Drc);
+    if (small_svd_ && !force_dist_ && ccap <= kSmall && mcap <= kSmallTall) {
+        // every column pair of a round on its own 16-lane group, matrix in LDS
+        hipLaunchKernelGGL(k_svd_small, dim3(1), dim3(kDistThreads), bytes, stream_, work_.p, ld, dims_.p, p, chi_,
+                           sv_.p, perm_.p, trunc_.p, stats_.p);
+        TQ_LAUNCH_CHECK();
+    } else if (bytes <= 144 * 1024 && !force_dist_) {
         hipLaunchKernelGGL(k_svd, dim3(1), dim3(kSvdThreads), bytes, stream_, work_.p, ld, dims_.p, p, chi_, sv_.p,
                            perm_.p, trunc_.p, 1, stats_.p);
         TQ_LAUNCH_CHECK();
     } else {
-        const int ccap = std::min(2 * Dlc, 2 * Drc), mcap = std::max(2 * Dlc, 2 * Drc);
         const int nb = (ccap + kBlk - 1) / kBlk;
         const int nbp = std::max(2, nb + (nb & 1));
         const unsigned nwg = (unsigned)(nbp / 2) * (unsigned)xcd_stride_;

@@ -133,6 +133,7 @@ private:
     long long timed_n_ = 0;
     int ld_ = 0;                   // leading dimension of mat_/work_
     bool force_dist_ = false;
+    bool small_svd_ = true;  // thetas up to 64 columns x 128 rows take k_svd_small
     int xcd_stride_ = 1;
     DevBuf<double> h_, bias_;
     DevBuf<double2> envA_, envB_, envT_;

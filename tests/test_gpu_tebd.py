@@ -161,6 +161,40 @@ def test_distributed_svd_path_matches(monkeypatch):
     assert np.array_equal(base["bonds"], dist["bonds"])
 
 
+@pytest.mark.parametrize("n,chi,steps", [(12, 64, 12), (10, 8, 25)])
+def test_small_svd_kernel_matches_workgroup_svd(monkeypatch, n, chi, steps):
+    """The per-update SVD of thetas up to 64 columns x 128 rows runs on 16-lane
+    column-pair groups (k_svd_small); the one-pair-per-wave workgroup SVD
+    (k_svd) gives the same run. At exact chi (n=12 chain, thetas up to 128 x 32
+    and 64 x 64, both row counts of the small kernel) the states agree to
+    rounding. A truncated run of this scheme (no canonical form) is
+    ill-conditioned: two LAPACK drivers (gesdd, gesvd) on the chi=8 case end
+    7.5e-3 apart in fidelity and 1.6e-2 in <H>, with discarded weights equal to
+    1.3e-4 relative. There the bar is identical bond dimensions and a discarded
+    weight within ten times the LAPACK spread."""
+    import scipy.linalg as sl
+    from tebd_numpy import tebd_numpy
+    model = I.chain_nnn(n, 70 + n)
+    sched = I.linear(steps)
+    small = _gpu_run(model, sched, 0.1, chi)
+    monkeypatch.setenv("TESSERA_SMALL_SVD", "0")
+    wg = _gpu_run(model, sched, 0.1, chi)
+    inf = 1 - _fid(small["state"], wg["state"])
+    print(f"n={n} chi={chi}: infidelity={inf:.2e} max bond={small['bonds'].max()} "
+          f"trunc {small['max_trunc']:.6e} / {wg['max_trunc']:.6e}", flush=True)
+    assert np.array_equal(small["bonds"], wg["bonds"])
+    if wg["max_trunc"] <= 1e-20:  # exact chi
+        assert inf <= 1e-12
+        assert np.abs(small["entropy"] - wg["entropy"]).max() <= 1e-10
+        assert np.array_equal(small["spins"], wg["spins"])
+        return
+    t_dd, _ = tebd_numpy(model, sched, 0.1, chi)
+    t_vd, _ = tebd_numpy(model, sched, 0.1, chi, svd=lambda a: sl.svd(a, full_matrices=False, lapack_driver="gesvd"))
+    spread = abs(t_dd - t_vd)
+    print(f"LAPACK trunc gesdd={t_dd:.6e} gesvd={t_vd:.6e} spread={spread:.2e}", flush=True)
+    assert abs(small["max_trunc"] - wg["max_trunc"]) <= 10 * spread + 1e-12
+
+
 def test_upstream_mps_suite():
     """cuda/tests/test_mps.cpp:78-152 through tessera_qa_anneal_ex."""
     sched = I.smooth(300)

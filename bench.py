@@ -11,9 +11,11 @@ Workload (one "step" = one symmetric Trotter step of the MPS-TEBD anneal):
   left-isometries with bond weights ~exp(-k/40), so every interior update
   factorises a 512 x 512 complex128 theta and keeps 256 of its singular values.
 
-By default the timed steps run in the lockstep executor (one cooperative launch
-in which consecutive Trotter steps overlap; --executor 0 runs one launch
-sequence per update instead).
+By default the timed steps (6) run in the lockstep executor: one cooperative
+launch in which consecutive Trotter steps overlap (1874 of a step's 1998 updates
+form one dependency chain, so a launch of K steps has ~K updates per super-step
+while the pipeline fills; 6 steps give 5.46 on average, a 1000-step anneal in
+launches of 64 steps 7.5). --executor 0 runs one launch sequence per update.
 
 Output: one JSON line. `value` is two-site updates per second over the timed
 steps (inputs resident on the device). `roofline` covers the dominant kernel
@@ -49,7 +51,8 @@ REF_SWEEPS_CHI256 = 25
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=6,
+                    help="timed Trotter steps (tebd: one launch; consecutive steps overlap in its super-steps)")
     ap.add_argument("--warmup", type=int, default=0,
                     help="untimed steps (the saturated-state preparation already runs every kernel)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
@@ -534,6 +537,7 @@ def main():
     sess.set_timing(True)
     st0 = sess.svd_stats()
     ex0 = sess.exec_stats()
+    pr0 = sess.exec_profile()
     t0 = time.perf_counter()
     if args.executor:
         sess.run(a[args.warmup:], b[args.warmup:])  # one launch: the steps overlap
@@ -558,8 +562,11 @@ def main():
         ex1 = sess.exec_stats()
         ss = ex1["supersteps"] - ex0["supersteps"]
         kernel = "k_tebd_lockstep (theta, lockstep block-Jacobi SVDs, reconstruction; Jacobi flops counted)"
+        pr1 = sess.exec_profile()
         timing = {"kernel_ms": svd_ms, "kernel_share_of_wall": svd_ms * 1e-3 / wall, "supersteps": ss,
-                  "updates_per_superstep": (ex1["updates"] - ex0["updates"]) / max(ss, 1)}
+                  "updates_per_superstep": (ex1["updates"] - ex0["updates"]) / max(ss, 1),
+                  "group_barriers": os.environ.get("TESSERA_EXEC_GROUP_SYNC", "0") == "1",
+                  "workgroup0_phases": {k: round(pr1[k] - pr0[k], 3) for k in pr1}}
     else:
         kernel = "one-sided Jacobi SVD (k_svd_dist / k_svd)"
         timing = {"svd_ms_per_update": svd_ms / max(calls, 1), "svd_share_of_wall": svd_ms * 1e-3 / wall}

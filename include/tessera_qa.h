@@ -206,6 +206,13 @@ int tessera_tebd_batch_run_ex(tessera_tebd* const* sessions, size_t count, const
 /* Lockstep executor totals since creation: super-steps run and two-site
  * updates placed in them (updates / (super-steps * groups) = slot fill). */
 int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* out_updates);
+/* Diagnostics (no upstream counterpart): phase profile of the lockstep
+ * executor's first workgroup since creation, first `count` of 12 slots: ms in
+ * 0 snapshots + theta, 1 LQ preconditioning, 2 Jacobi sweeps, 3 whole SVD,
+ * 4 wait after the SVD, 5 reconstruction, 6 wait at the super-step end,
+ * 8 waits at tournament-round barriers, 10 whole launches; counts in 7
+ * super-steps with an update, 9 tournament rounds, 11 Jacobi sweeps. */
+int tessera_tebd_exec_profile(tessera_tebd* t, double* out, size_t count);
 /* Number of two-site updates one Trotter step performs for this model. */
 size_t tessera_tebd_updates_per_step(const tessera_tebd* t);
 void tessera_tebd_destroy(tessera_tebd* t);

@@ -500,6 +500,19 @@ int tessera_tebd_batch_run_ex(tessera_tebd* const* sessions, size_t count, const
     }
 }
 
+int tessera_tebd_exec_profile(tessera_tebd* t, double* out, size_t count) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        if (!out && count) throw Error(TESSERA_ERR_ARG, "null output");
+        const std::vector<double> prof = t->eng->exec_profile();
+        for (size_t q = 0; q < count && q < prof.size(); ++q) out[q] = prof[q];
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
 int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* out_updates) {
     try {
         if (!t) throw Error(TESSERA_ERR_ARG, "null session");

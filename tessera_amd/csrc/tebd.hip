@@ -342,10 +342,17 @@ constexpr int kDistThreads = 512;  // 8 waves: one column pair per wave
 constexpr int kStatSlots = 1024;   // per-workgroup accounting slots (largest grid)
 
 // Shared LDS of the block-Jacobi SVD (static part).
+// Executor phase profile of workgroup 0, in wall-clock ticks (slot meanings
+// at k_tebd_lockstep).
+constexpr int kProfSlots = 12;
+
 struct SvdShared {
+    unsigned long long tm[kProfSlots];  // phase profile (workgroup 0, thread 0)
     double nrm[kMaxCols];
     int perm[kMaxCols];
     double bn[kBlk];  // norms^2 of the LDS block's columns
+    double cn[64];    // svd_small: carried norms^2 of the LDS columns
+    double cr[64];    // svd_small: each column's norm^2 when last computed exactly
     int rot;
     unsigned long long vis, rots;
     double2 red[kDistThreads / 64][16];  // LQ: per-wave partial row dots
@@ -539,6 +546,7 @@ struct SvdMode {
     int* gflags = nullptr;      // grid-wide rotation flags, [2]
     bool grid_leader = false;   // clears gflags
     bool active = true;         // false: this group has no matrix
+    bool prof = false;          // this workgroup records the phase profile in sh.tm
 };
 
 // Block-Jacobi SVD run by `nworkers` workgroups that synchronise with `sync()`.
@@ -583,9 +591,16 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
         }
     sync();
     const double floor2 = noise_floor2(__hip_atomic_load(fro, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const bool prof = mode.prof && tid == 0;
+    unsigned long long tp = prof ? wall_clock64() : 0;
     if (aux) {  // W <- L of W = L Q^H (same left singular vectors and values)
         const int panels = mode.lq_panels >= 0 ? mode.lq_panels : (c + 15) / 16;
         lq_precondition(Wg, ld, m, c, aux, taus, me, nworkers, sync, lds, sh, panels, mode.active);
+    }
+    if (prof) {
+        const unsigned long long t = wall_clock64();
+        sh.tm[1] += t - tp;
+        tp = t;
     }
     bool working = mode.active;
     int sweeps_done = 0;
@@ -680,7 +695,12 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 }
                 __syncthreads();  // the next pair of this worker reuses the LDS columns
             }
+            const unsigned long long tw = prof ? wall_clock64() : 0;
             sync();
+            if (prof) {
+                sh.tm[8] += wall_clock64() - tw;
+                sh.tm[9] += 1;
+            }
         }
         const int f = __hip_atomic_load(&flags[sweep & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int gf = mode.gflags ? __hip_atomic_load(&mode.gflags[sweep & 1], __ATOMIC_RELAXED,
@@ -691,6 +711,10 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
         if (mode.grid_leader && tid == 0 && mode.gflags) mode.gflags[sweep & 1] = 0;
         if (!f) working = false;
         if (!gf) break;
+    }
+    if (prof) {
+        sh.tm[2] += wall_clock64() - tp;
+        sh.tm[11] += (unsigned long long)sweeps_done;
     }
     if (lane == 0 && nvis) {
         atomicAdd(&sh.vis, nvis * (unsigned long long)m);  // LDS
@@ -814,6 +838,21 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
     for (int sweep = 0; sweep < 60 && cp >= 2; ++sweep) {
         ++sweeps;
         if (tid == 0) sh.rot = 0;
+        // column norms are computed exactly once per sweep and carried across
+        // its rotations (|x'|^2 = |x|^2 - t|g|, |y'|^2 = |y|^2 + t|g|), each
+        // recomputed once it has dropped 64x below its last exact value (the
+        // rounding of the updates stays below ~64 * 63 eps relative), so a
+        // visit reduces only the two parts of the pair's inner product
+        for (int col = grp; col < c; col += ngrp) {
+            const double2* pc = A + (std::size_t)col * m;
+            double s2 = 0.0;
+            for (int row = l16; row < m; row += 16) s2 += cnorm(pc[row]);
+            s2 = sum16(s2);
+            if (l16 == 0) {
+                sh.cn[col] = s2;
+                sh.cr[col] = s2;
+            }
+        }
         __syncthreads();
         for (int r = 0; r < cp - 1; ++r) {
             for (int k = grp; k < cp / 2; k += ngrp) {
@@ -823,19 +862,15 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
                 double2* px = A + (std::size_t)i * m;
                 double2* py = A + (std::size_t)j * m;
                 double2 x[RPL], y[RPL];
-                double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+                double al = sh.cn[i], be = sh.cn[j], gr = 0.0, gi = 0.0;
 #pragma unroll
                 for (int q = 0; q < RPL; ++q) {
                     const int row = l16 + 16 * q;
                     x[q] = row < m ? px[row] : make_double2(0, 0);
                     y[q] = row < m ? py[row] : make_double2(0, 0);
-                    al += cnorm(x[q]);
-                    be += cnorm(y[q]);
                     gr += x[q].x * y[q].x + x[q].y * y[q].y;  // conj(x) y
                     gi += x[q].x * y[q].y - x[q].y * y[q].x;
                 }
-                al = sum16(al);
-                be = sum16(be);
                 gr = sum16(gr);
                 gi = sum16(gi);
                 ++nvis;
@@ -847,14 +882,30 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
                 const double zeta = (be - al) * (0.5 * ig);
                 const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                 const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                double nx = 0.0, ny = 0.0;
 #pragma unroll
                 for (int q = 0; q < RPL; ++q) {
                     const int row = l16 + 16 * q;
                     const double2 u = x[q], v = cmul(y[q], ph);
+                    x[q] = make_double2(cs * u.x - sn * v.x, cs * u.y - sn * v.y);
+                    y[q] = make_double2(sn * u.x + cs * v.x, sn * u.y + cs * v.y);
+                    nx += cnorm(x[q]);
+                    ny += cnorm(y[q]);
                     if (row < m) {
-                        px[row] = make_double2(cs * u.x - sn * v.x, cs * u.y - sn * v.y);
-                        py[row] = make_double2(sn * u.x + cs * v.x, sn * u.y + cs * v.y);
+                        px[row] = x[q];
+                        py[row] = y[q];
                     }
+                }
+                double al1 = al - t * gabs, be1 = be + t * gabs;
+                // uniform in the group (group-wide values), so the shuffles read active lanes
+                const bool rx = al1 < sh.cr[i] * (1.0 / 64), ry = be1 < sh.cr[j] * (1.0 / 64);
+                if (rx) al1 = sum16(nx);
+                if (ry) be1 = sum16(ny);
+                if (l16 == 0) {
+                    sh.cn[i] = al1;
+                    sh.cn[j] = be1;
+                    if (rx) sh.cr[i] = al1;
+                    if (ry) sh.cr[j] = be1;
                 }
                 ++nrot;
                 if (l16 == 0) sh.rot = 1;
@@ -1365,6 +1416,46 @@ struct ExecArgs {
     double2* taus;      // [G][ld]
     int solo;           // 1: one workgroup per group, workgroup barriers inside an update
     int small;          // solo: theta up to kSmall x kSmall takes svd_small (dynamic LDS fits it)
+    int group_sync;     // grouped: each group's SVD on its own barrier (else the grid barrier, in lockstep)
+    int* group_bar;     // [G][2] arrival count, generation
+    int* err;           // raised by a GroupSync wait that gave up
+    unsigned long long* prof;  // [kProfSlots] phase profile of workgroup 0 (accumulated)
+};
+
+// Barrier of the nwg co-resident workgroups of one executor group (the
+// cooperative launch keeps every workgroup resident). bar[0] counts arrivals,
+// bar[1] is the generation: the last arriver resets the count and advances the
+// generation, the others poll it. Every wait ends: when the generation moves,
+// when another workgroup has raised *err, or after kSpinLimit polls, which
+// raise *err (the host reports it) instead of hanging the device.
+constexpr long long kSpinLimit = 1ll << 24;
+struct GroupSync {
+    int* bar;
+    int nwg;
+    int* err;
+    __device__ void operator()() const {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();  // this workgroup's writes, before the arrival
+            const int gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int prev = __hip_atomic_fetch_add(bar, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == nwg - 1) {
+                __hip_atomic_store(bar, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(bar + 1, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                for (long long spins = 0;
+                     __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen; ++spins) {
+                    if (spins >= kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __threadfence();  // the other workgroups' writes, after the release
+        }
+        __syncthreads();
+    }
 };
 
 // Barrier of one workgroup for code written against a barrier functor: the
@@ -1376,6 +1467,10 @@ struct BlockSync {
     }
 };
 
+// GS: the grouped layout runs each group's update on the group's own barrier
+// (GroupSync); otherwise all groups' SVDs advance in lockstep on the grid
+// barrier. One instantiation per barrier kind keeps each within its registers.
+template <bool GS>
 __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
     extern __shared__ double2 lds[];
     __shared__ SvdShared sh;
@@ -1396,6 +1491,22 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
         a.gflags[0] = 0;
         a.gflags[1] = 0;
     }
+    // phase profile of workgroup 0 (wall-clock ticks): 0 snapshots + theta,
+    // 1 LQ preconditioning, 2 Jacobi sweeps, 3 whole SVD, 4 wait after the SVD,
+    // 5 reconstruction, 6 wait at the super-step end, 7 super-steps with an
+    // update, 8 waits at tournament-round barriers, 9 rounds, 10 kernel, 11 sweeps
+    const bool prof = blockIdx.x == 0 && tid == 0;
+    if (prof)
+        for (int q = 0; q < kProfSlots; ++q) sh.tm[q] = 0;
+    const unsigned long long t_start = prof ? wall_clock64() : 0;
+    unsigned long long tp = t_start;
+    auto mark = [&](int slot) {
+        if (prof) {
+            const unsigned long long t = wall_clock64();
+            sh.tm[slot] += t - tp;
+            tp = t;
+        }
+    };
     const ExecInst& I0 = a.insts[0];
     for (int st = 0; st < a.nsteps; ++st) {
         const bool active = grp < a.up_begin[st + 1] - a.up_begin[st];
@@ -1443,20 +1554,55 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
             sync();  // the super-step's sites are final
             continue;
         }
-        if (active) theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
-        // 2-3. lockstep SVD; svd_blocks opens with a grid barrier
-        SvdMode mode;
-        mode.rounds = a.rounds[st];
-        mode.gflags = a.gflags;
-        mode.grid_leader = blockIdx.x == 0;
-        mode.active = active;
-        mode.lq_panels = (a.ld + 15) / 16;
-        svd_blocks(W, a.ld, I.dims, active ? u.site : 0, I.chi, sv, perm, I.trunc, sflags, I.stats, rank, nwg, sync,
-                   lds, sh, mode, a.aux + grp * mat_sz, a.taus + (std::size_t)grp * a.ld);
-        sync();
-        // 4. reconstruction
-        if (active) recon_part(I.sites, I.off, I.cap, I.dims, u.site, M, W, a.ld, sv, perm, gid, gstride);
-        sync();
+        if constexpr (GS) {
+            // the snapshots read sites that this super-step's reconstructions may rewrite
+            if (a.snap_begin[st + 1] > a.snap_begin[st]) sync();
+            if (active) {
+                // theta, SVD and reconstruction of this group's update on the
+                // group's own barrier: own tournament length and convergence
+                GroupSync gs{a.group_bar + 2 * grp, nwg, a.err};
+                theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
+                mark(0);
+                SvdMode mode;
+                mode.prof = blockIdx.x == 0;
+                svd_blocks(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, rank, nwg, gs, lds, sh,
+                           mode, a.aux + grp * mat_sz, a.taus + (std::size_t)grp * a.ld);
+                mark(3);
+                gs();
+                mark(4);
+                recon_part(I.sites, I.off, I.cap, I.dims, u.site, M, W, a.ld, sv, perm, gid, gstride);
+                mark(5);
+                if (prof) sh.tm[7] += 1;
+            }
+            sync();  // the super-step's sites are final
+            mark(6);
+        } else {
+            if (active) theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
+            mark(0);
+            // 2-3. lockstep SVD; svd_blocks opens with a grid barrier
+            SvdMode mode;
+            mode.rounds = a.rounds[st];
+            mode.gflags = a.gflags;
+            mode.grid_leader = blockIdx.x == 0;
+            mode.active = active;
+            mode.lq_panels = (a.ld + 15) / 16;
+            mode.prof = blockIdx.x == 0;
+            svd_blocks(W, a.ld, I.dims, active ? u.site : 0, I.chi, sv, perm, I.trunc, sflags, I.stats, rank, nwg, sync,
+                       lds, sh, mode, a.aux + grp * mat_sz, a.taus + (std::size_t)grp * a.ld);
+            mark(3);
+            sync();
+            mark(4);
+            // 4. reconstruction
+            if (active) recon_part(I.sites, I.off, I.cap, I.dims, u.site, M, W, a.ld, sv, perm, gid, gstride);
+            mark(5);
+            if (prof && active) sh.tm[7] += 1;
+            sync();
+            mark(6);
+        }
+    }
+    if (prof) {
+        sh.tm[10] += wall_clock64() - t_start;
+        for (int q = 0; q < kProfSlots; ++q) a.prof[q] += sh.tm[q];
     }
 }
 
@@ -1540,13 +1686,21 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     TQ_HIP(hipMemcpy(off_dev_.p, off_.data(), (n_ + 1) * sizeof(std::size_t), hipMemcpyHostToDevice));
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd_dist, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
-    TQ_HIP(hipFuncSetAttribute((const void*)k_tebd_lockstep, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
+    TQ_HIP(hipFuncSetAttribute((const void*)k_tebd_lockstep<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               144 * 1024));
+    TQ_HIP(hipFuncSetAttribute((const void*)k_tebd_lockstep<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               144 * 1024));
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd_small, hipFuncAttributeMaxDynamicSharedMemorySize,
                                kSmall * kSmallTall * (int)sizeof(double2)));
     const char* fd = std::getenv("TESSERA_FORCE_DIST_SVD");  // test knob: distributed SVD at every size
     force_dist_ = fd && fd[0] == '1';
     const char* ss = std::getenv("TESSERA_SMALL_SVD");  // test knob: 0 keeps small thetas on k_svd
     small_svd_ = !(ss && ss[0] == '0');
+    // knob: 1 runs each group's update of the grouped executor on the group's
+    // own barrier instead of all groups' SVDs in lockstep on the grid barrier
+    // (same results; slower on the n = 128, chi = 256 chain)
+    const char* gs = std::getenv("TESSERA_EXEC_GROUP_SYNC");
+    ex_group_sync_ = gs && gs[0] == '1';
     const char* xp = std::getenv("TESSERA_SVD_XCD_PACK");  // 1: keep the SVD's workgroups on one XCD
     xcd_stride_ = (xp && xp[0] == '1') ? 8 : 1;
     build_plan();
@@ -1778,7 +1932,11 @@ static void exec_shape(int maxcap, int req, bool solo, int workers, int& groups,
     int per_cu = 0, cus = 0, dev = 0;
     TQ_HIP(hipGetDevice(&dev));
     TQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_tebd_lockstep, kDistThreads, shm));
+    int per_cu_gs = 0;
+    TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_tebd_lockstep<false>, kDistThreads, shm));
+    TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_gs, (const void*)k_tebd_lockstep<true>, kDistThreads,
+                                                        shm));
+    per_cu = std::min(per_cu, per_cu_gs);  // either instantiation may be launched
     const long long resident = std::min<long long>((long long)per_cu * cus, kStatSlots);
     long long g = resident / per_group;
     const long long cap = req > 0 ? req : (solo ? kStatSlots : 64);
@@ -1986,6 +2144,11 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
         t0.ex_taus_.alloc((std::size_t)G * ld);
     }
     if (t0.ex_flags_.n < 4 * (std::size_t)G + 2) t0.ex_flags_.alloc(4 * (std::size_t)G + 2);
+    if (t0.ex_bar_.n < 2 * (std::size_t)G + 1) t0.ex_bar_.alloc(2 * (std::size_t)G + 1);
+    if (!t0.ex_prof_.p) {
+        t0.ex_prof_.alloc(kProfSlots);
+        TQ_HIP(hipMemsetAsync(t0.ex_prof_.p, 0, kProfSlots * sizeof(unsigned long long), t0.stream_));
+    }
     const int count0 = counts[0];
     if (rec0 >= 0 && t0.ex_snap_.n < (std::size_t)count0 * t0.off_[t0.n_]) {
         t0.ex_snap_.alloc((std::size_t)count0 * t0.off_[t0.n_]);
@@ -2041,8 +2204,16 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.snap_dims = t0.ex_sdims_.p;
     x.aux = t0.ex_aux_.p;
     x.taus = t0.ex_taus_.p;
+    x.group_sync = (!solo && t0.ex_group_sync_) ? 1 : 0;
+    x.err = t0.ex_bar_.p;  // the error word, then [G][2] barrier words
+    x.group_bar = t0.ex_bar_.p + 1;
+    x.prof = t0.ex_prof_.p;
     void* args[] = {(void*)&x};
     hipStream_t st = t0.stream_;
+    // barrier counters and the error word start at zero (ordinary stream work:
+    // this launch sequence is never captured into a graph)
+    TQ_HIP(hipMemsetAsync(t0.ex_bar_.p, 0, (2 * (std::size_t)G + 1) * sizeof(int), st));
+    t0.ex_armed_ = true;
     if (t0.timing_) {
         if (t0.ev_next_ >= t0.events_.size()) {
             hipEvent_t e0, e1;
@@ -2053,7 +2224,9 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
         TQ_HIP(hipEventRecord(t0.events_[t0.ev_next_].first, st));
     }
     if (total_steps > 0)
-        TQ_HIP(hipLaunchCooperativeKernel((const void*)k_tebd_lockstep, dim3((unsigned)(G * W)), dim3(kDistThreads),
+        TQ_HIP(hipLaunchCooperativeKernel(x.group_sync ? (const void*)k_tebd_lockstep<true>
+                                                       : (const void*)k_tebd_lockstep<false>,
+                                          dim3((unsigned)(G * W)), dim3(kDistThreads),
                                           args, shm, st));
     if (t0.timing_) TQ_HIP(hipEventRecord(t0.events_[t0.ev_next_++].second, st));
     t0.exec_supersteps_ += total_steps;
@@ -2157,6 +2330,28 @@ void Tebd::sync() {
         ++timed_n_;
     }
     ev_next_ = 0;
+    if (ex_armed_) {  // a group barrier of the last executor launch that gave up
+        ex_armed_ = false;
+        int err = 0;
+        TQ_HIP(hipMemcpy(&err, ex_bar_.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (err) throw Error(-3, "lockstep executor: a group barrier wait timed out; results are invalid");
+    }
+}
+
+std::vector<double> Tebd::exec_profile() {
+    std::vector<double> out(kProfSlots, 0.0);
+    if (!ex_prof_.p) return out;
+    std::vector<unsigned long long> h(kProfSlots);
+    TQ_HIP(hipMemcpyAsync(h.data(), ex_prof_.p, kProfSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          stream_));
+    sync();
+    int dev = 0, khz = 0;
+    TQ_HIP(hipGetDevice(&dev));
+    TQ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    const double ms_per_tick = khz > 0 ? 1.0 / khz : 0.0;
+    for (int q = 0; q < kProfSlots; ++q)
+        out[q] = (q == 7 || q == 9 || q == 11) ? (double)h[q] : (double)h[q] * ms_per_tick;
+    return out;
 }
 
 void Tebd::set_timing(bool on) {

@@ -75,6 +75,9 @@ public:
     void set_bias(const double* bias);
     double max_trunc();
     SvdStats svd_stats();
+    // Executor phase profile of workgroup 0 since creation, in ms (slots 0-6,
+    // 8, 10) or counts (7 super-steps with an update, 9 rounds, 11 sweeps).
+    std::vector<double> exec_profile();
     void set_timing(bool on);  // HIP events around every SVD launch
     std::vector<int> bonds();                    // n+1 bond dims
     struct Readout {
@@ -149,6 +152,10 @@ private:
     DevBuf<double2> ex_mat_, ex_work_, ex_snap_, ex_gates_, ex_aux_, ex_taus_;
     DevBuf<double> ex_sv_;
     DevBuf<int> ex_perm_, ex_flags_, ex_sdims_;
+    DevBuf<int> ex_bar_;                   // the barrier error word, then group barriers [G][2]
+    DevBuf<unsigned long long> ex_prof_;   // phase profile of workgroup 0
+    bool ex_group_sync_ = false;           // grouped layout: per-group barriers inside an update
+    bool ex_armed_ = false;                // a launch whose barrier error word is still unread
     std::vector<unsigned char> ex_host_;
 };
 

@@ -92,6 +92,8 @@ def lib() -> ctypes.CDLL:
         L.tessera_tebd_batch_run_ex.argtypes = [vp, _usz, vp, vp, vp, ctypes.c_int, ctypes.c_int]
         L.tessera_tebd_exec_stats.restype = ctypes.c_int
         L.tessera_tebd_exec_stats.argtypes = [vp, vp, vp]
+        L.tessera_tebd_exec_profile.restype = ctypes.c_int
+        L.tessera_tebd_exec_profile.argtypes = [vp, vp, ctypes.c_size_t]
         L.tessera_tebd_reset_warm.restype = ctypes.c_int
         L.tessera_tebd_reset_warm.argtypes = [vp, vp]
         L.tessera_tebd_expect_z.restype = ctypes.c_int
@@ -388,6 +390,16 @@ class TebdSession:
         up = ctypes.c_int64()
         _check(lib().tessera_tebd_exec_stats(self._h, ctypes.addressof(ss), ctypes.addressof(up)))
         return dict(supersteps=ss.value, updates=up.value)
+
+    PROFILE_SLOTS = ("theta_ms", "lq_ms", "jacobi_ms", "svd_ms", "post_svd_wait_ms", "recon_ms",
+                     "superstep_wait_ms", "active_supersteps", "round_barrier_ms", "rounds", "kernel_ms", "sweeps")
+
+    def exec_profile(self) -> dict:
+        """Phase profile of the lockstep executor's first workgroup since creation
+        (ms, or counts for active_supersteps, rounds and sweeps)."""
+        out = np.zeros(len(self.PROFILE_SLOTS))
+        _check(lib().tessera_tebd_exec_profile(self._h, _p(out), out.size))
+        return dict(zip(self.PROFILE_SLOTS, out.tolist()))
 
     @property
     def updates_per_step(self) -> int:

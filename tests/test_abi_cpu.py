@@ -105,6 +105,7 @@ def test_session_entries_reject_null_sessions():
     L = qa.lib()
     assert L.tessera_tebd_set_executor(None, 0) == -1
     assert L.tessera_tebd_exec_stats(None, None, None) == -1
+    assert L.tessera_tebd_exec_profile(None, None, 0) == -1
     r = np.zeros(1, np.int32)
     assert L.tessera_tebd_theta(None, 0, None, 0, qa._p(r), qa._p(r)) == -1
     assert L.tessera_tebd_batch_run(None, 1, None, None, None, -1) == -1

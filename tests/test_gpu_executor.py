@@ -165,3 +165,30 @@ def test_group_barriers_match_lockstep_grid_barrier(monkeypatch, chi, workers):
     assert inf <= 1e-12
     assert np.abs(grp["entropy"] - lock["entropy"]).max() <= 1e-10
     assert abs(grp["max_trunc"] - lock["max_trunc"]) <= 1e-12 + 1e-9 * lock["max_trunc"]
+
+
+def test_exec_profile_phases_are_consistent():
+    """tessera_tebd_exec_profile: workgroup 0's phase timers partition each launch
+    (snapshots + theta, SVD, wait after it, reconstruction, super-step-end wait),
+    the LQ, Jacobi and round-barrier times lie inside the SVD time, and the totals
+    accumulate over launches."""
+    model = I.chain_range(14, 3, 29)
+    sched = I.linear(4)
+    s = qa.TebdSession(model, 32, 0.1)
+    s.set_executor(-1)
+    s.run(sched.a[:2], sched.b[:2])
+    s.sync()
+    p1 = s.exec_profile()
+    s.run(sched.a[2:], sched.b[2:])
+    s.sync()
+    p2 = s.exec_profile()
+    s.close()
+    print({k: round(v, 3) for k, v in p2.items()}, flush=True)
+    assert p1["kernel_ms"] > 0 and p1["rounds"] > 0 and p1["sweeps"] > 0 and p1["active_supersteps"] > 0
+    parts = p1["theta_ms"] + p1["svd_ms"] + p1["post_svd_wait_ms"] + p1["recon_ms"] + p1["superstep_wait_ms"]
+    assert parts <= p1["kernel_ms"] * (1 + 1e-9) and parts >= 0.95 * p1["kernel_ms"]
+    assert p1["jacobi_ms"] + p1["lq_ms"] <= p1["svd_ms"] * (1 + 1e-9)
+    assert p1["round_barrier_ms"] <= p1["jacobi_ms"]
+    for k in p1:
+        assert p2[k] >= p1[k]
+    assert p2["kernel_ms"] > p1["kernel_ms"] and p2["rounds"] > p1["rounds"]

@@ -86,13 +86,27 @@ def progress(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_info() -> dict:
+    """CPU model and logical core count of the host the baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu": model, "logical_cores": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(chi: int):
     """Bounded sample of the upstream update on one host core (bit-exact port)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle  # the CPU oracle: only the baseline leg uses it
 
     t_setup, sweeps, t_sweep = pyoracle.update_probe(chi, 1)
-    info = {"cores": 1, "kind": "port"}
+    info = {"cores": 1, "kind": "port", "host": host_info()}
     if REF_SWEEPS_CHI256 and chi == 256:
         per_update = t_setup + REF_SWEEPS_CHI256 * t_sweep
         info.update(value=1.0 / per_update, unit="two-site updates/s",

@@ -1414,8 +1414,7 @@ struct ExecArgs {
     int* snap_dims;     // [slots][n + 1]
     double2* aux;       // [G][ld * ld] LQ reflectors
     double2* taus;      // [G][ld]
-    int solo;           // 1: one workgroup per group, workgroup barriers inside an update
-    int small;          // solo: theta up to kSmall x kSmall takes svd_small (dynamic LDS fits it)
+    int small;          // solo layout: thetas within kSmall x kSmall take svd_small (dynamic LDS fits it)
     int group_sync;     // grouped: each group's SVD on its own barrier (else the grid barrier, in lockstep)
     int* group_bar;     // [G][2] arrival count, generation
     int* err;           // raised by a GroupSync wait that gave up
@@ -1467,10 +1466,12 @@ struct BlockSync {
     }
 };
 
-// GS: the grouped layout runs each group's update on the group's own barrier
-// (GroupSync); otherwise all groups' SVDs advance in lockstep on the grid
-// barrier. One instantiation per barrier kind keeps each within its registers.
-template <bool GS>
+// Layouts, one instantiation each so that every one stays within its registers:
+// kExecLockstep (grouped, all groups' SVDs advance in lockstep on the grid
+// barrier), kExecGroupSync (grouped, each group's update on the group's own
+// barrier, GroupSync), kExecSolo (one workgroup per update).
+constexpr int kExecLockstep = 0, kExecGroupSync = 1, kExecSolo = 2;
+template <int LAYOUT>
 __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
     extern __shared__ double2 lds[];
     __shared__ SvdShared sh;
@@ -1532,7 +1533,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 sd[i + 1] = Dr;
             }
         }
-        if (a.solo) {
+        if constexpr (LAYOUT == kExecSolo) {
             // the snapshots read sites that this super-step's reconstructions may rewrite
             if (a.snap_begin[st + 1] > a.snap_begin[st]) sync();
             if (active) {
@@ -1541,8 +1542,12 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 bs();
                 const int R = 2 * I.dims[u.site], C = 2 * I.dims[u.site + 2];
                 const int m = C <= R ? R : C, c = C <= R ? C : R;
+                // thetas within the kSmall x kSmall LDS area: 64 x 64, or up to 128 x 32
                 if (a.small && m <= kSmall && c <= kSmall) {
                     svd_small<kSmallRows>(W, a.ld, I.dims, u.site, I.chi, m, c, sv, perm, I.trunc, I.stats, lds, sh);
+                } else if (a.small && m <= kSmallTall && c <= kSmall && m * c <= kSmall * kSmall) {
+                    svd_small<kSmallTall / 16>(W, a.ld, I.dims, u.site, I.chi, m, c, sv, perm, I.trunc, I.stats, lds,
+                                               sh);
                 } else {
                     SvdMode mode;  // own tournament length, own convergence
                     svd_blocks(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, 0, 1, bs, lds,
@@ -1552,9 +1557,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 recon_part(I.sites, I.off, I.cap, I.dims, u.site, M, W, a.ld, sv, perm, tid, blockDim.x);
             }
             sync();  // the super-step's sites are final
-            continue;
-        }
-        if constexpr (GS) {
+        } else if constexpr (LAYOUT == kExecGroupSync) {
             // the snapshots read sites that this super-step's reconstructions may rewrite
             if (a.snap_begin[st + 1] > a.snap_begin[st]) sync();
             if (active) {
@@ -1686,10 +1689,9 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     TQ_HIP(hipMemcpy(off_dev_.p, off_.data(), (n_ + 1) * sizeof(std::size_t), hipMemcpyHostToDevice));
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd_dist, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
-    TQ_HIP(hipFuncSetAttribute((const void*)k_tebd_lockstep<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               144 * 1024));
-    TQ_HIP(hipFuncSetAttribute((const void*)k_tebd_lockstep<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               144 * 1024));
+    for (const void* k : {(const void*)k_tebd_lockstep<kExecLockstep>, (const void*)k_tebd_lockstep<kExecGroupSync>,
+                          (const void*)k_tebd_lockstep<kExecSolo>})
+        TQ_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     TQ_HIP(hipFuncSetAttribute((const void*)k_svd_small, hipFuncAttributeMaxDynamicSharedMemorySize,
                                kSmall * kSmallTall * (int)sizeof(double2)));
     const char* fd = std::getenv("TESSERA_FORCE_DIST_SVD");  // test knob: distributed SVD at every size
@@ -1917,7 +1919,13 @@ void Tebd::set_executor(int groups, int workers) {
 // a theta with rows and columns up to 2*maxcap), as many groups as are
 // co-resident (at most `req` if req > 0, else 64), a multiple of 8 when 8 or
 // more fit.
-static void exec_shape(int maxcap, int req, bool solo, int workers, int& groups, int& per_group, std::size_t& shm) {
+static const void* exec_kernel(bool solo, bool group_sync) {
+    if (solo) return (const void*)k_tebd_lockstep<kExecSolo>;
+    return group_sync ? (const void*)k_tebd_lockstep<kExecGroupSync> : (const void*)k_tebd_lockstep<kExecLockstep>;
+}
+
+static void exec_shape(int maxcap, int req, bool solo, bool group_sync, int workers, int& groups, int& per_group,
+                       std::size_t& shm) {
     const int cmax = 2 * maxcap;
     const int nb = (cmax + kBlk - 1) / kBlk;
     const int nbp = std::max(2, nb + (nb & 1));
@@ -1932,11 +1940,7 @@ static void exec_shape(int maxcap, int req, bool solo, int workers, int& groups,
     int per_cu = 0, cus = 0, dev = 0;
     TQ_HIP(hipGetDevice(&dev));
     TQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    int per_cu_gs = 0;
-    TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_tebd_lockstep<false>, kDistThreads, shm));
-    TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_gs, (const void*)k_tebd_lockstep<true>, kDistThreads,
-                                                        shm));
-    per_cu = std::min(per_cu, per_cu_gs);  // either instantiation may be launched
+    TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, exec_kernel(solo, group_sync), kDistThreads, shm));
     const long long resident = std::min<long long>((long long)per_cu * cus, kStatSlots);
     long long g = resident / per_group;
     const long long cap = req > 0 ? req : (solo ? kStatSlots : 64);
@@ -1973,7 +1977,8 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     const bool solo = mode == 2 || (mode == 0 && ts.size() >= 4 && maxcap <= 32);
     int G = 0, W = 0;
     std::size_t shm = 0;
-    exec_shape(maxcap, req, solo, ts[0]->exec_workers_, G, W, shm);
+    const bool group_sync = !solo && t0.ex_group_sync_;
+    exec_shape(maxcap, req, solo, group_sync, ts[0]->exec_workers_, G, W, shm);
     using cd = std::complex<double>;
     struct M2 {
         cd u[2][2];
@@ -2187,7 +2192,6 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.rounds = reinterpret_cast<const int*>(t0.ex_tasks_.p + in_bytes + up_bytes + sn_bytes);
     x.snap_begin = x.rounds + total_steps;
     x.up_begin = x.rounds + 2 * total_steps + 1;
-    x.solo = solo ? 1 : 0;
     x.small = solo ? 1 : 0;
     x.nsteps = total_steps;
     x.G = G;
@@ -2204,7 +2208,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.snap_dims = t0.ex_sdims_.p;
     x.aux = t0.ex_aux_.p;
     x.taus = t0.ex_taus_.p;
-    x.group_sync = (!solo && t0.ex_group_sync_) ? 1 : 0;
+    x.group_sync = group_sync ? 1 : 0;
     x.err = t0.ex_bar_.p;  // the error word, then [G][2] barrier words
     x.group_bar = t0.ex_bar_.p + 1;
     x.prof = t0.ex_prof_.p;
@@ -2224,8 +2228,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
         TQ_HIP(hipEventRecord(t0.events_[t0.ev_next_].first, st));
     }
     if (total_steps > 0)
-        TQ_HIP(hipLaunchCooperativeKernel(x.group_sync ? (const void*)k_tebd_lockstep<true>
-                                                       : (const void*)k_tebd_lockstep<false>,
+        TQ_HIP(hipLaunchCooperativeKernel(exec_kernel(solo, group_sync),
                                           dim3((unsigned)(G * W)), dim3(kDistThreads),
                                           args, shm, st));
     if (t0.timing_) TQ_HIP(hipEventRecord(t0.events_[t0.ev_next_++].second, st));

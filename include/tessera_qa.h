@@ -198,7 +198,7 @@ int tessera_tebd_batch_run(tessera_tebd* const* sessions, size_t count, const do
  * grouped (each update's SVD spread over a group of workgroups that advance in
  * lockstep: few, large updates), 2 = solo (one workgroup per update, one grid
  * barrier per super-step, up to one update per resident workgroup, thetas up
- * to 64 x 64 factorised in LDS: many independent small updates, as in batches
+ * to 64 x 64 or 128 x 32 factorised in LDS: many independent small updates, as in batches
  * of small instances). groups caps the
  * concurrent updates (-1: as many as fit). Results agree to rounding. */
 int tessera_tebd_batch_run_ex(tessera_tebd* const* sessions, size_t count, const double* const* a,

@@ -125,6 +125,11 @@ __device__ inline void rr_pair(int r, int k, int cp, int& i, int& j) {
     }
 }
 
+// Sort key of a column norm for the stable descending rank: NaN ranks below
+// every number, so the ranks always form a permutation of the columns (the
+// permutation indexes the working matrix) whatever the data.
+__device__ inline double rank_key(double x) { return x == x ? x : -1.0; }
+
 // Rows held per lane for m <= 512.
 constexpr int kRowsPerLane = 8;
 
@@ -277,10 +282,10 @@ __global__ void __launch_bounds__(kSvdThreads) k_svd(double2* __restrict__ Wg, i
     }
     __syncthreads();
     if (tid < c) {  // stable descending rank
-        const double v = s_nrm[tid];
+        const double v = rank_key(s_nrm[tid]);
         int rank = 0;
         for (int q = 0; q < c; ++q) {
-            const double u = s_nrm[q];
+            const double u = rank_key(s_nrm[q]);
             rank += (u > v) || (u == v && q < tid);
         }
         s_perm[rank] = tid;
@@ -739,10 +744,10 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     for (int q = tid; q < c; q += blockDim.x) sh.nrm[q] = sv_out[q];
     __syncthreads();
     for (int q = tid; q < c; q += blockDim.x) {
-        const double v = sh.nrm[q];
+        const double v = rank_key(sh.nrm[q]);
         int rank = 0;
         for (int o = 0; o < c; ++o) {
-            const double u = sh.nrm[o];
+            const double u = rank_key(sh.nrm[o]);
             rank += (u > v) || (u == v && o < q);
         }
         sh.perm[rank] = q;
@@ -812,11 +817,11 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
     __syncthreads();
     double fro = 0.0;
     for (int q = tid; q < c; q += blockDim.x) {
-        const double v = sh.nrm[q];
+        const double v = rank_key(sh.nrm[q]);
         fro += v;
         int rank = 0;
         for (int o = 0; o < c; ++o) {
-            const double u = sh.nrm[o];
+            const double u = rank_key(sh.nrm[o]);
             rank += (u > v) || (u == v && o < q);
         }
         sh.perm[rank] = q;
@@ -934,10 +939,10 @@ __device__ void svd_small(double2* __restrict__ Wg, int ld, int* dims, int p, in
     }
     __syncthreads();
     for (int q = tid; q < c; q += blockDim.x) {  // stable descending rank
-        const double v = sh.nrm[q];
+        const double v = rank_key(sh.nrm[q]);
         int rank = 0;
         for (int o = 0; o < c; ++o) {
-            const double u = sh.nrm[o];
+            const double u = rank_key(sh.nrm[o]);
             rank += (u > v) || (u == v && o < q);
         }
         sh.perm[rank] = q;
@@ -1418,6 +1423,7 @@ struct ExecArgs {
     int group_sync;     // grouped: each group's SVD on its own barrier (else the grid barrier, in lockstep)
     int* group_bar;     // [G][2] arrival count, generation
     int* err;           // raised by a GroupSync wait that gave up
+    long long spin_limit;  // GroupSync polls before giving up
     unsigned long long* prof;  // [kProfSlots] phase profile of workgroup 0 (accumulated)
 };
 
@@ -1432,6 +1438,7 @@ struct GroupSync {
     int* bar;
     int nwg;
     int* err;
+    long long limit;  // polls before giving up (kSpinLimit unless a test lowers it)
     __device__ void operator()() const {
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1444,7 +1451,7 @@ struct GroupSync {
             } else {
                 for (long long spins = 0;
                      __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen; ++spins) {
-                    if (spins >= kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    if (spins >= limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
                     }
@@ -1563,7 +1570,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
             if (active) {
                 // theta, SVD and reconstruction of this group's update on the
                 // group's own barrier: own tournament length and convergence
-                GroupSync gs{a.group_bar + 2 * grp, nwg, a.err};
+                GroupSync gs{a.group_bar + 2 * grp, nwg, a.err, a.spin_limit};
                 theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
                 mark(0);
                 SvdMode mode;
@@ -1703,6 +1710,9 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     // (same results; slower on the n = 128, chi = 256 chain)
     const char* gs = std::getenv("TESSERA_EXEC_GROUP_SYNC");
     ex_group_sync_ = gs && gs[0] == '1';
+    // test knob: polls before a group barrier wait gives up (forces the error path)
+    const char* sl = std::getenv("TESSERA_EXEC_SPIN_LIMIT");
+    ex_spin_limit_ = sl ? std::max(0ll, std::atoll(sl)) : kSpinLimit;
     const char* xp = std::getenv("TESSERA_SVD_XCD_PACK");  // 1: keep the SVD's workgroups on one XCD
     xcd_stride_ = (xp && xp[0] == '1') ? 8 : 1;
     build_plan();
@@ -2210,6 +2220,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.taus = t0.ex_taus_.p;
     x.group_sync = group_sync ? 1 : 0;
     x.err = t0.ex_bar_.p;  // the error word, then [G][2] barrier words
+    x.spin_limit = t0.ex_spin_limit_;
     x.group_bar = t0.ex_bar_.p + 1;
     x.prof = t0.ex_prof_.p;
     void* args[] = {(void*)&x};

@@ -156,6 +156,7 @@ private:
     DevBuf<unsigned long long> ex_prof_;   // phase profile of workgroup 0
     bool ex_group_sync_ = false;           // grouped layout: per-group barriers inside an update
     bool ex_armed_ = false;                // a launch whose barrier error word is still unread
+    long long ex_spin_limit_ = 1ll << 24;  // GroupSync polls before a wait gives up
     std::vector<unsigned char> ex_host_;
 };
 

@@ -192,3 +192,25 @@ def test_exec_profile_phases_are_consistent():
     for k in p1:
         assert p2[k] >= p1[k]
     assert p2["kernel_ms"] > p1["kernel_ms"] and p2["rounds"] > p1["rounds"]
+
+
+def test_group_barrier_wait_that_gives_up_is_reported(monkeypatch):
+    """Every wait of the per-group barrier is bounded: a wait that gives up raises
+    the executor's error word, and the session reports it instead of returning
+    results. Forced from the host with a limit of zero polls (each wait not
+    released at its first poll gives up at once, later waits stop on the error
+    word, and the launch still ends at its grid barriers)."""
+    monkeypatch.setenv("TESSERA_EXEC_GROUP_SYNC", "1")
+    monkeypatch.setenv("TESSERA_EXEC_SPIN_LIMIT", "0")
+    model = I.chain_range(12, 3, 23)
+    sched = I.linear(2)
+    s = qa.TebdSession(model, 32, 0.1)
+    s.set_executor(8)
+    with pytest.raises(qa.TesseraError, match="timed out"):
+        s.run(sched.a, sched.b, record=False)
+        s.sync()
+    s.close()
+    # a fresh session with the default limit runs normally afterwards
+    monkeypatch.delenv("TESSERA_EXEC_SPIN_LIMIT")
+    g = _run(model, sched, 0.1, 32, 8)
+    assert np.isfinite(g["max_trunc"]) and g["bonds"].max() <= 32

@@ -575,11 +575,14 @@ def main():
     if args.executor:
         ex1 = sess.exec_stats()
         ss = ex1["supersteps"] - ex0["supersteps"]
-        kernel = "k_tebd_lockstep (theta, lockstep block-Jacobi SVDs, reconstruction; Jacobi flops counted)"
+        kernel = ("k_tebd_lockstep (super-steps of theta, block-Jacobi SVDs on per-group barriers with "
+                  "point-to-point tournament rounds, reconstruction; Jacobi flops counted)")
         pr1 = sess.exec_profile()
         timing = {"kernel_ms": svd_ms, "kernel_share_of_wall": svd_ms * 1e-3 / wall, "supersteps": ss,
                   "updates_per_superstep": (ex1["updates"] - ex0["updates"]) / max(ss, 1),
-                  "group_barriers": os.environ.get("TESSERA_EXEC_GROUP_SYNC", "0") == "1",
+                  "group_barriers": os.environ.get("TESSERA_EXEC_LOCKSTEP", "0") != "1",
+                  "point_to_point_rounds": (os.environ.get("TESSERA_EXEC_LOCKSTEP", "0") != "1"
+                                            and os.environ.get("TESSERA_EXEC_P2P", "1") != "0"),
                   "workgroup0_phases": {k: round(pr1[k] - pr0[k], 3) for k in pr1}}
     else:
         kernel = "one-sided Jacobi SVD (k_svd_dist / k_svd)"

@@ -552,7 +552,35 @@ struct SvdMode {
     bool grid_leader = false;   // clears gflags
     bool active = true;         // false: this group has no matrix
     bool prof = false;          // this workgroup records the phase profile in sh.tm
+    // point-to-point rounds (one block pair per worker): per-worker round counters,
+    // [nworkers] x 64 ints; null: a full barrier after every round
+    int* epochs = nullptr;
+    int* err = nullptr;         // raised by a round wait that gave up
+    long long spin_limit = 0;   // polls before a round wait gives up
 };
+
+// Worker holding block b in tournament round r of rr_pair with one pair per
+// worker: block cp - 1 and block r belong to pair 0, any other block to the pair
+// k in [1, cp/2 - 1] with b = r + k or b = r - k (mod cp - 1).
+__device__ inline int rr_holder(int r, int b, int cp) {
+    const int L = cp - 1;
+    if (b == L || b == r) return 0;
+    const int k1 = ((b - r) % L + L) % L;
+    return (k1 >= 1 && k1 <= cp / 2 - 1) ? k1 : ((r - b) % L + L) % L;
+}
+
+// Thread 0 of a workgroup waits until a round counter reaches `target`; every
+// wait ends (counter reached, another wait gave up, or the poll limit, which
+// raises *err).
+__device__ inline void wait_counter(int* ctr, int target, int* err, long long limit) {
+    for (long long spins = 0; __hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target; ++spins) {
+        if (spins >= limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
 
 // Block-Jacobi SVD run by `nworkers` workgroups that synchronise with `sync()`.
 // Worker `me` takes block pairs me, me + nworkers, ... of each tournament round
@@ -608,11 +636,32 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
         tp = t;
     }
     bool working = mode.active;
+    // Point-to-point rounds: inside a sweep, round r + 1 of worker me needs only
+    // the two blocks that their holders wrote back in round r, so instead of a
+    // barrier each worker publishes a round counter (release) and waits for its
+    // two producers' counters (acquire). Each block's home columns are written
+    // only by its current holder and read only by the next one. The last round of
+    // a sweep keeps the full barrier (the rotation flags are read after it).
+    const bool p2p = mode.epochs != nullptr && nworkers == P && mode.active;
+    int ep = p2p ? __hip_atomic_load(&mode.epochs[(std::size_t)me * 64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     int sweeps_done = 0;
     unsigned long long nvis = 0, nrot = 0;
     for (int sweep = 0; sweep < 40; ++sweep) {
         if (working) ++sweeps_done;
         for (int r = 0; r < rounds; ++r) {
+            if (p2p && r > 0) {
+                const unsigned long long tw = prof ? wall_clock64() : 0;
+                if (tid == 0) {
+                    int bi, bj;
+                    rr_pair(r, me, nbp, bi, bj);
+                    const int pa = rr_holder(r - 1, bi, nbp), pb = rr_holder(r - 1, bj, nbp);
+                    if (pa != me) wait_counter(&mode.epochs[(std::size_t)pa * 64], ep, mode.err, mode.spin_limit);
+                    if (pb != me) wait_counter(&mode.epochs[(std::size_t)pb * 64], ep, mode.err, mode.spin_limit);
+                    __threadfence();  // the producers' blocks, after their release
+                }
+                __syncthreads();
+                if (prof) sh.tm[8] += wall_clock64() - tw;
+            }
             // worker `me` takes pairs me, me + nworkers, ... of the round (one
             // pair when there are P workers, all of them for a lone workgroup)
             if (working && r < nbp - 1)
@@ -700,11 +749,28 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 }
                 __syncthreads();  // the next pair of this worker reuses the LDS columns
             }
-            const unsigned long long tw = prof ? wall_clock64() : 0;
-            sync();
-            if (prof) {
-                sh.tm[8] += wall_clock64() - tw;
-                sh.tm[9] += 1;
+            if (p2p && r < rounds - 1) {
+                // publish this round's blocks: every wave waits until its write-back
+                // stores are complete in the L2 (s_waitcnt of all counters; stores
+                // count in vmcnt on CDNA), the workgroup barrier follows, and thread 0's
+                // agent-scope release then writes the L2 back before it stores the
+                // counter (its own fence orders only its own stores)
+                ++ep;
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                if (tid == 0) {
+                    __threadfence();
+                    __hip_atomic_store(&mode.epochs[(std::size_t)me * 64], ep, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (prof) sh.tm[9] += 1;
+            } else {
+                const unsigned long long tw = prof ? wall_clock64() : 0;
+                sync();
+                if (prof) {
+                    sh.tm[8] += wall_clock64() - tw;
+                    sh.tm[9] += 1;
+                }
             }
         }
         const int f = __hip_atomic_load(&flags[sweep & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1424,6 +1490,7 @@ struct ExecArgs {
     int* group_bar;     // [G][2] arrival count, generation
     int* err;           // raised by a GroupSync wait that gave up
     long long spin_limit;  // GroupSync polls before giving up
+    int* epochs;        // [G][W][64] round counters of the point-to-point rounds (per-group layout)
     unsigned long long* prof;  // [kProfSlots] phase profile of workgroup 0 (accumulated)
 };
 
@@ -1440,9 +1507,10 @@ struct GroupSync {
     int* err;
     long long limit;  // polls before giving up (kSpinLimit unless a test lowers it)
     __device__ void operator()() const {
+        __builtin_amdgcn_s_waitcnt(0);  // every wave's stores complete in the L2 before the arrival
         __syncthreads();
         if (threadIdx.x == 0) {
-            __threadfence();  // this workgroup's writes, before the arrival
+            __threadfence();  // thread 0 releases them (with the L2) at agent scope
             const int gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const int prev = __hip_atomic_fetch_add(bar, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             if (prev == nwg - 1) {
@@ -1575,6 +1643,9 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 mark(0);
                 SvdMode mode;
                 mode.prof = blockIdx.x == 0;
+                mode.epochs = a.epochs ? a.epochs + (std::size_t)grp * nwg * 64 : nullptr;
+                mode.err = a.err;
+                mode.spin_limit = a.spin_limit;
                 svd_blocks(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, rank, nwg, gs, lds, sh,
                            mode, a.aux + grp * mat_sz, a.taus + (std::size_t)grp * a.ld);
                 mark(3);
@@ -1705,11 +1776,14 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     force_dist_ = fd && fd[0] == '1';
     const char* ss = std::getenv("TESSERA_SMALL_SVD");  // test knob: 0 keeps small thetas on k_svd
     small_svd_ = !(ss && ss[0] == '0');
-    // knob: 1 runs each group's update of the grouped executor on the group's
-    // own barrier instead of all groups' SVDs in lockstep on the grid barrier
+    // knob: 1 runs all groups' SVDs of the grouped executor in lockstep on the
+    // grid barrier instead of each group's update on the group's own barrier
     // (same results; slower on the n = 128, chi = 256 chain)
-    const char* gs = std::getenv("TESSERA_EXEC_GROUP_SYNC");
-    ex_group_sync_ = gs && gs[0] == '1';
+    const char* ls = std::getenv("TESSERA_EXEC_LOCKSTEP");
+    ex_group_sync_ = !(ls && ls[0] == '1');
+    // knob: 0 keeps the per-group layout on a full barrier after every round
+    const char* pp = std::getenv("TESSERA_EXEC_P2P");
+    ex_p2p_ = !(pp && pp[0] == '0');
     // test knob: polls before a group barrier wait gives up (forces the error path)
     const char* sl = std::getenv("TESSERA_EXEC_SPIN_LIMIT");
     ex_spin_limit_ = sl ? std::max(0ll, std::atoll(sl)) : kSpinLimit;
@@ -2221,6 +2295,14 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.group_sync = group_sync ? 1 : 0;
     x.err = t0.ex_bar_.p;  // the error word, then [G][2] barrier words
     x.spin_limit = t0.ex_spin_limit_;
+    if (group_sync && t0.ex_p2p_) {
+        const std::size_t ne = (std::size_t)G * W * 64;
+        if (t0.ex_ep_.n < ne) t0.ex_ep_.alloc(ne);
+        TQ_HIP(hipMemsetAsync(t0.ex_ep_.p, 0, ne * sizeof(int), t0.stream_));
+        x.epochs = t0.ex_ep_.p;
+    } else {
+        x.epochs = nullptr;
+    }
     x.group_bar = t0.ex_bar_.p + 1;
     x.prof = t0.ex_prof_.p;
     void* args[] = {(void*)&x};

@@ -145,26 +145,33 @@ def test_executor_n128_matches_stream():
 
 @pytest.mark.parametrize("chi,workers", [(32, 0), (8, 2)])
 def test_group_barriers_match_lockstep_grid_barrier(monkeypatch, chi, workers):
-    """The grouped layout runs all groups' SVDs in lockstep on the grid barrier
-    (the extra rounds and panels of a smaller matrix doing no work); with
-    TESSERA_EXEC_GROUP_SYNC=1 each group's update runs on the group's own
-    barrier (own tournament length, LQ panel count and convergence). Each
-    matrix sees the same rotations, so both give the same run: a range-3 chain
-    of 12 sites with up to 8 updates per super-step, exact at chi = 32 and
-    truncated at chi = 8 (two workers per group, several block pairs each)."""
+    """The grouped layout runs each group's update on the group's own barrier
+    (own tournament length, LQ panel count and convergence), with point-to-point
+    tournament rounds when every worker holds one block pair; the knobs switch
+    the rounds back to full per-group barriers (TESSERA_EXEC_P2P=0) or run all
+    groups' SVDs in lockstep on the grid barrier (TESSERA_EXEC_LOCKSTEP=1, the
+    extra rounds and panels of a smaller matrix doing no work). Each matrix sees
+    the same rotations, so all three give the same run: a range-3 chain of 12
+    sites with up to 8 updates per super-step, exact at chi = 32 (4 workers, one
+    pair each: point-to-point rounds) and truncated at chi = 8 (two workers per
+    group, several block pairs each: full barriers)."""
     model = I.chain_range(12, 3, 23)
     sched = I.linear(5)
-    lock = _run(model, sched, 0.1, chi, 8, workers=workers)
-    monkeypatch.setenv("TESSERA_EXEC_GROUP_SYNC", "1")
     grp = _run(model, sched, 0.1, chi, 8, workers=workers)
-    inf = 1 - _fid(grp["state"], lock["state"])
-    same = np.array_equal(grp["state"], lock["state"])
-    print(f"chi={chi} workers={workers}: infidelity={inf:.2e} bit-identical={same} "
-          f"trunc {grp['max_trunc']:.6e} / {lock['max_trunc']:.6e}", flush=True)
-    assert np.array_equal(grp["bonds"], lock["bonds"])
-    assert inf <= 1e-12
-    assert np.abs(grp["entropy"] - lock["entropy"]).max() <= 1e-10
-    assert abs(grp["max_trunc"] - lock["max_trunc"]) <= 1e-12 + 1e-9 * lock["max_trunc"]
+    monkeypatch.setenv("TESSERA_EXEC_P2P", "0")
+    full = _run(model, sched, 0.1, chi, 8, workers=workers)
+    monkeypatch.delenv("TESSERA_EXEC_P2P")
+    monkeypatch.setenv("TESSERA_EXEC_LOCKSTEP", "1")
+    lock = _run(model, sched, 0.1, chi, 8, workers=workers)
+    for name, other in (("full barriers", full), ("lockstep", lock)):
+        inf = 1 - _fid(grp["state"], other["state"])
+        same = np.array_equal(grp["state"], other["state"])
+        print(f"chi={chi} workers={workers} vs {name}: infidelity={inf:.2e} bit-identical={same} "
+              f"trunc {grp['max_trunc']:.6e} / {other['max_trunc']:.6e}", flush=True)
+        assert np.array_equal(grp["bonds"], other["bonds"])
+        assert inf <= 1e-12
+        assert np.abs(grp["entropy"] - other["entropy"]).max() <= 1e-10
+        assert abs(grp["max_trunc"] - other["max_trunc"]) <= 1e-12 + 1e-9 * other["max_trunc"]
 
 
 def test_exec_profile_phases_are_consistent():
@@ -200,7 +207,6 @@ def test_group_barrier_wait_that_gives_up_is_reported(monkeypatch):
     results. Forced from the host with a limit of zero polls (each wait not
     released at its first poll gives up at once, later waits stop on the error
     word, and the launch still ends at its grid barriers)."""
-    monkeypatch.setenv("TESSERA_EXEC_GROUP_SYNC", "1")
     monkeypatch.setenv("TESSERA_EXEC_SPIN_LIMIT", "0")
     model = I.chain_range(12, 3, 23)
     sched = I.linear(2)

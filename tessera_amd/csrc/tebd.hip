@@ -19,6 +19,7 @@
 // engine computes the same factorisation accurately, which is what the upstream
 // algorithm defines.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdlib>
@@ -1444,9 +1445,10 @@ __device__ void theta_tiles(const double2* __restrict__ sites, const std::size_t
 // touching it changes the site (sites change only in phase 4, bond dims at the
 // end of phase 3), with the single-site gates still pending applied.
 struct ExecUpdate {
-    int site;   // -1: idle slot
-    int inst;   // instance (MPS) the update belongs to
-    Gate4 g;    // two-site gate with pending single-site gates folded in
+    int site;     // -1: idle slot
+    int inst;     // instance (MPS) the update belongs to
+    int pred[2];  // latest earlier updates on sites site, site + 1 (flat indices; -1: none)
+    Gate4 g;      // two-site gate with pending single-site gates folded in
 };
 // One MPS advanced by the executor (several in a batched run).
 struct ExecInst {
@@ -1491,6 +1493,9 @@ struct ExecArgs {
     int* err;           // raised by a GroupSync wait that gave up
     long long spin_limit;  // GroupSync polls before giving up
     int* epochs;        // [G][W][64] round counters of the point-to-point rounds (per-group layout)
+    int dataflow;       // per-group layout without snapshots: no super-step barriers; an update
+                        // waits for the done flags of its predecessors instead
+    int* done;          // [updates] done flags of the dataflow order
     unsigned long long* prof;  // [kProfSlots] phase profile of workgroup 0 (accumulated)
 };
 
@@ -1634,8 +1639,19 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
             sync();  // the super-step's sites are final
         } else if constexpr (LAYOUT == kExecGroupSync) {
             // the snapshots read sites that this super-step's reconstructions may rewrite
-            if (a.snap_begin[st + 1] > a.snap_begin[st]) sync();
+            if (!a.dataflow && a.snap_begin[st + 1] > a.snap_begin[st]) sync();
             if (active) {
+                if (a.dataflow) {
+                    // the sites of this update are final once the latest earlier
+                    // updates on them are done (each of those waited for its own)
+                    if (tid == 0) {
+                        for (int q = 0; q < 2; ++q)
+                            if (u.pred[q] >= 0) wait_counter(&a.done[u.pred[q]], 1, a.err, a.spin_limit);
+                        __threadfence();
+                    }
+                    __syncthreads();
+                    mark(6);
+                }
                 // theta, SVD and reconstruction of this group's update on the
                 // group's own barrier: own tournament length and convergence
                 GroupSync gs{a.group_bar + 2 * grp, nwg, a.err, a.spin_limit};
@@ -1654,9 +1670,20 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 recon_part(I.sites, I.off, I.cap, I.dims, u.site, M, W, a.ld, sv, perm, gid, gstride);
                 mark(5);
                 if (prof) sh.tm[7] += 1;
+                if (a.dataflow) {
+                    gs();  // every workgroup's part of the reconstruction is released
+                    if (rank == 0 && tid == 0) {
+                        __threadfence();
+                        __hip_atomic_store(&a.done[a.up_begin[st] + grp], 1, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    mark(4);
+                }
             }
-            sync();  // the super-step's sites are final
-            mark(6);
+            if (!a.dataflow) {
+                sync();  // the super-step's sites are final
+                mark(6);
+            }
         } else {
             if (active) theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
             mark(0);
@@ -1784,6 +1811,10 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     // knob: 0 keeps the per-group layout on a full barrier after every round
     const char* pp = std::getenv("TESSERA_EXEC_P2P");
     ex_p2p_ = !(pp && pp[0] == '0');
+    // knob: 1 drops the super-step barriers of the per-group layout (dataflow order:
+    // same results; no faster on the n = 128 chain, slower on mixed-chi batches)
+    const char* df = std::getenv("TESSERA_EXEC_DATAFLOW");
+    ex_dataflow_ = df && df[0] == '1';
     // test knob: polls before a group barrier wait gives up (forces the error path)
     const char* sl = std::getenv("TESSERA_EXEC_SPIN_LIMIT");
     ex_spin_limit_ = sl ? std::max(0ll, std::atoll(sl)) : kSpinLimit;
@@ -2078,6 +2109,9 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     auto from_d2 = [](double2 z) { return cd(z.x, z.y); };
     const Gate4 sw = make_swap_gate();
     std::vector<std::vector<ExecUpdate>> ups;  // per super-step, at most G
+    // per update (parallel to ups): (super-step << 16 | slot) of its predecessors on
+    // its two sites, -1: none; flat update indices once the offsets are known
+    std::vector<std::vector<std::array<long long, 2>>> pred_pos;
     std::vector<int> rounds;
     struct SnapAt {
         int at;
@@ -2126,10 +2160,14 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
         pending.assign(n, eye);
         std::vector<int> free_at(n, 0);   // first super-step at which the site may be updated
         std::vector<int> last_at(n, -1);  // super-step of the site's latest update
+        // (super-step << 16 | slot) of the site's latest update, -1: none; converted
+        // to flat update indices once every super-step's offset is known
+        std::vector<long long> last_pos(n, -1);
         auto place = [&](int p, const Gate4& g) {
             const int t = first_open(std::max(free_at[p], free_at[p + 1]));
             if (t >= (int)ups.size()) {
                 ups.resize(t + 1);
+                pred_pos.resize(t + 1);
                 rounds.resize(t + 1, 0);
                 for (int q = (int)open.size(); q <= t; ++q) open.push_back(q);
             }
@@ -2137,6 +2175,10 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
             u.site = p;
             u.inst = (int)in;
             u.g = g;
+            const long long pa = last_pos[p], pb = last_pos[p + 1];
+            u.pred[0] = u.pred[1] = -1;
+            pred_pos[t].push_back({pa, pb == pa ? -1 : pb});
+            last_pos[p] = last_pos[p + 1] = ((long long)t << 16) | (long long)ups[t].size();
             ups[t].push_back(u);
             if ((int)ups[t].size() >= G) open[t] = t + 1;  // full: later searches skip it
             rounds[t] = std::max(rounds[t], nbp_of(p) - 1);
@@ -2256,6 +2298,12 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     t0.ex_host_.assign(in_bytes + up_bytes + sn_bytes + int_count * sizeof(int), 0);
     unsigned char* hb = t0.ex_host_.data();
     std::memcpy(hb, insts.data(), insts.size() * sizeof(ExecInst));
+    for (int t = 0; t < nsteps; ++t)  // predecessors as flat update indices
+        for (std::size_t k = 0; k < ups[t].size(); ++k)
+            for (int q = 0; q < 2; ++q) {
+                const long long e = pred_pos[t][k][q];
+                ups[t][k].pred[q] = e < 0 ? -1 : up_begin[(int)(e >> 16)] + (int)(e & 0xffff);
+            }
     for (int t = 0; t < total_steps; ++t)
         if (!ups[t].empty())
             std::memcpy(hb + in_bytes + (std::size_t)up_begin[t] * sizeof(ExecUpdate), ups[t].data(),
@@ -2295,6 +2343,14 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.group_sync = group_sync ? 1 : 0;
     x.err = t0.ex_bar_.p;  // the error word, then [G][2] barrier words
     x.spin_limit = t0.ex_spin_limit_;
+    x.dataflow = (group_sync && t0.ex_dataflow_ && snap_list.empty()) ? 1 : 0;
+    x.done = nullptr;
+    if (x.dataflow) {
+        const std::size_t nu = std::max<std::size_t>(1, up_begin[total_steps]);
+        if (t0.ex_done_.n < nu) t0.ex_done_.alloc(nu);
+        TQ_HIP(hipMemsetAsync(t0.ex_done_.p, 0, nu * sizeof(int), t0.stream_));
+        x.done = t0.ex_done_.p;
+    }
     if (group_sync && t0.ex_p2p_) {
         const std::size_t ne = (std::size_t)G * W * 64;
         if (t0.ex_ep_.n < ne) t0.ex_ep_.alloc(ne);

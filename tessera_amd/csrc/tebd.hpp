@@ -159,6 +159,8 @@ private:
     long long ex_spin_limit_ = 1ll << 24;  // GroupSync polls before a wait gives up
     bool ex_p2p_ = true;                   // per-group layout: point-to-point tournament rounds
     DevBuf<int> ex_ep_;                    // their round counters
+    bool ex_dataflow_ = false;             // per-group layout: updates wait for their predecessors only
+    DevBuf<int> ex_done_;                  // their done flags
     std::vector<unsigned char> ex_host_;
 };
 

@@ -220,3 +220,31 @@ def test_group_barrier_wait_that_gives_up_is_reported(monkeypatch):
     monkeypatch.delenv("TESSERA_EXEC_SPIN_LIMIT")
     g = _run(model, sched, 0.1, 32, 8)
     assert np.isfinite(g["max_trunc"]) and g["bonds"].max() <= 32
+
+
+@pytest.mark.parametrize("chi", [32, 8])
+def test_dataflow_order_matches_superstep_barriers(monkeypatch, chi):
+    """With TESSERA_EXEC_DATAFLOW=1 and no entropy recording the per-group
+    layout drops the super-step barriers: each update waits only for the done
+    flags of the latest earlier updates on its two sites (every dependency, bond
+    dimensions included, goes through a shared site). The same rotations run on
+    the same data, so the run equals the default one with super-step barriers
+    and the lockstep one: a range-3 chain of 14 sites, 5 steps in one launch,
+    exact at chi = 32 and truncated at chi = 8."""
+    model = I.chain_range(14, 3, 31)
+    sched = I.linear(5)
+    steps = _run(model, sched, 0.1, chi, 8, record=False)
+    monkeypatch.setenv("TESSERA_EXEC_DATAFLOW", "1")
+    flow = _run(model, sched, 0.1, chi, 8, record=False)
+    monkeypatch.delenv("TESSERA_EXEC_DATAFLOW")
+    monkeypatch.setenv("TESSERA_EXEC_LOCKSTEP", "1")
+    lock = _run(model, sched, 0.1, chi, 8, record=False)
+    for name, other in (("super-step barriers", steps), ("lockstep", lock)):
+        inf = 1 - _fid(flow["state"], other["state"])
+        same = np.array_equal(flow["state"], other["state"])
+        print(f"chi={chi} vs {name}: infidelity={inf:.2e} bit-identical={same} "
+              f"trunc {flow['max_trunc']:.6e} / {other['max_trunc']:.6e}", flush=True)
+        assert np.array_equal(flow["bonds"], other["bonds"])
+        assert inf <= 1e-12
+        assert abs(flow["max_trunc"] - other["max_trunc"]) <= 1e-12 + 1e-9 * other["max_trunc"]
+        assert np.array_equal(flow["spins"], other["spins"])

@@ -670,18 +670,19 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 int bi, bj;
                 rr_pair(r, pi, nbp, bi, bj);
                 if (tid == 0) sh.rot = 0;
-                // LDS column q <-> block (q < 8 ? bi : bj), member q & 7
+                double2 xa[kRowsPerLane], ya[kRowsPerLane];
+                // LDS column q <-> block (q < 8 ? bi : bj), member q & 7; a column's
+                // rows go through registers with all of a lane's loads in flight at
+                // once (a load-store loop would wait out each load in turn)
                 for (int q = wave; q < 2 * kBlk; q += kDistThreads / 64) {
                     const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
-                    double2* dst = lds + (std::size_t)q * m;
                     if (col < c) {
-                        const double2* src = Wg + (std::size_t)col * ld;
-                        for (int row = lane; row < m; row += 64) dst[row] = src[row];
+                        load_col(xa, Wg + (std::size_t)col * ld, m, lane);
+                        store_col(xa, lds + (std::size_t)q * m, m, lane);
                     }
                 }
                 __syncthreads();
                 bool rotated = false;
-                double2 xa[kRowsPerLane], ya[kRowsPerLane];
                 if (r == 0) {  // pairs inside each block, once per sweep
                     for (int s = 0; s < kBlk - 1; ++s) {
                         const int half = wave >> 2, k = wave & 3;  // waves 0-3 block bi, 4-7 block bj
@@ -742,10 +743,9 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 }
                 for (int q = wave; q < 2 * kBlk; q += kDistThreads / 64) {
                     const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
-                    const double2* src = lds + (std::size_t)q * m;
                     if (col < c) {
-                        double2* dst = Wg + (std::size_t)col * ld;
-                        for (int row = lane; row < m; row += 64) dst[row] = src[row];
+                        load_col(xa, lds + (std::size_t)q * m, m, lane);
+                        store_col(xa, Wg + (std::size_t)col * ld, m, lane);
                     }
                 }
                 __syncthreads();  // the next pair of this worker reuses the LDS columns
@@ -1098,39 +1098,50 @@ __device__ void recon_part(double2* sites, const std::size_t* off, const int* ca
     double2* Cn = sites + off[p + 1];
     const long long ndirect = tall ? (long long)R * keep : (long long)keep * C;
     const long long ngemm = tall ? (long long)keep * C : (long long)R * keep;
-    for (long long idx = gid; idx < ndirect + ngemm; idx += stride) {
-        if (idx < ndirect) {
-            if (tall) {
-                const int row = (int)(idx / keep), k = (int)(idx % keep);
-                const double s = sv[k];
-                const double2 w = W[(std::size_t)perm[k] * ld + row];
-                A[(std::size_t)row * sA + k] = s > 0 ? cscale(w, 1.0 / s) : make_double2(0, 0);
-            } else {
-                const int k = (int)(idx / C), col = (int)(idx % C);
-                const int s2 = col / Dr, b = col % Dr;
-                const double2 v = cconj(W[(std::size_t)perm[k] * ld + col]);
-                Cn[((std::size_t)k * 2 + s2) * sC + b] =
-                    wts ? (sv[k] > 0 ? cscale(v, wts[k] / sv[k]) : make_double2(0, 0)) : v;
-            }
+    for (long long idx = gid; idx < ndirect; idx += stride) {
+        if (tall) {
+            const int row = (int)(idx / keep), k = (int)(idx % keep);
+            const double s = sv[k];
+            const double2 w = W[(std::size_t)perm[k] * ld + row];
+            A[(std::size_t)row * sA + k] = s > 0 ? cscale(w, 1.0 / s) : make_double2(0, 0);
         } else {
-            const long long g = idx - ndirect;
-            if (tall) {
-                const int k = (int)(g / C), col = (int)(g % C);
+            const int k = (int)(idx / C), col = (int)(idx % C);
+            const int s2 = col / Dr, b = col % Dr;
+            const double2 v = cconj(W[(std::size_t)perm[k] * ld + col]);
+            Cn[((std::size_t)k * 2 + s2) * sC + b] =
+                wts ? (sv[k] > 0 ? cscale(v, wts[k] / sv[k]) : make_double2(0, 0)) : v;
+        }
+    }
+    // products, one output per wave: the lanes take every 64th term of the dot
+    // product (coalesced rows of both columns) and the wave sums them (gid and
+    // stride are thread indices in whole waves)
+    const int lane = (int)(gid & 63);
+    for (long long g = gid >> 6; g < ngemm; g += stride >> 6) {
+        double2 acc = make_double2(0, 0);
+        if (tall) {
+            const int k = (int)(g / C), col = (int)(g % C);
+            const double2* u = W + (std::size_t)perm[k] * ld;
+            const double2* mc = M + (std::size_t)col * ld;
+#pragma unroll 2
+            for (int row = lane; row < R; row += 64) acc = cadd(acc, cmulc(u[row], mc[row]));
+            acc.x = wave_sum(acc.x);
+            acc.y = wave_sum(acc.y);
+            if (lane == 0) {
                 const double s = sv[k];
-                const double2* u = W + (std::size_t)perm[k] * ld;
-                const double2* mc = M + (std::size_t)col * ld;
-                double2 acc = make_double2(0, 0);
-                for (int row = 0; row < R; ++row) acc = cadd(acc, cmulc(u[row], mc[row]));
                 const int s2 = col / Dr, b = col % Dr;
                 const double f = s > 0 ? (wts ? wts[k] / (s * s) : 1.0 / s) : 0.0;
                 Cn[((std::size_t)k * 2 + s2) * sC + b] = cscale(acc, f);
-            } else {
-                const int row = (int)(g / keep), k = (int)(g % keep);
+            }
+        } else {
+            const int row = (int)(g / keep), k = (int)(g % keep);
+            const double2* w = W + (std::size_t)perm[k] * ld;
+            const double2* mr = M + (std::size_t)row * ld;
+#pragma unroll 2
+            for (int col = lane; col < C; col += 64) acc = cadd(acc, cmulc(mr[col], w[col]));
+            acc.x = wave_sum(acc.x);
+            acc.y = wave_sum(acc.y);
+            if (lane == 0) {
                 const double s = sv[k];
-                const double2* w = W + (std::size_t)perm[k] * ld;
-                const double2* mr = M + (std::size_t)row * ld;
-                double2 acc = make_double2(0, 0);
-                for (int col = 0; col < C; ++col) acc = cadd(acc, cmulc(mr[col], w[col]));
                 A[(std::size_t)row * sA + k] = s > 0 ? cscale(acc, 1.0 / (s * s)) : make_double2(0, 0);
             }
         }

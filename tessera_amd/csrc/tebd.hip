@@ -381,9 +381,23 @@ struct SvdShared {
 // (rr = tid & 15, g = tid >> 4) owns row rr, columns jj = g, g + 32, ...
 __device__ __forceinline__ void lq_load_tile(const double2* __restrict__ Wg, int ld, double2* lds, int t, int r0,
                                              int n, int m) {
-    for (int idx = threadIdx.x; idx < n * 16; idx += blockDim.x) {
-        const int jj = idx >> 4, rr = idx & 15, row = t * 16 + rr;
-        lds[idx] = row < m ? Wg[(std::size_t)(r0 + jj) * ld + row] : make_double2(0, 0);
+    // four loads in flight per thread before their LDS stores (a load-store
+    // loop would wait out each load in turn)
+    constexpr int kBatch = 4;
+    const int total = n * 16;
+    for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
+        double2 v[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const int idx = base + q * blockDim.x;
+            const int jj = idx >> 4, rr = idx & 15, row = t * 16 + rr;
+            v[q] = (idx < total && row < m) ? Wg[(std::size_t)(r0 + jj) * ld + row] : make_double2(0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const int idx = base + q * blockDim.x;
+            if (idx < total) lds[idx] = v[q];
+        }
     }
     __syncthreads();
 }

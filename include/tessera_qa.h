@@ -195,8 +195,8 @@ int tessera_tebd_batch_run(tessera_tebd* const* sessions, size_t count, const do
                            const double* const* b, const size_t* steps, int groups);
 /* tessera_tebd_batch_run with the executor layout chosen explicitly: mode 0 =
  * automatic (solo for four or more sessions with chi <= 32, grouped otherwise), 1 =
- * grouped (each update's SVD spread over a group of workgroups that advance in
- * lockstep: few, large updates), 2 = solo (one workgroup per update, one grid
+ * grouped (each update's SVD spread over a group of workgroups that synchronise
+ * on the group's own barrier: few, large updates), 2 = solo (one workgroup per update, one grid
  * barrier per super-step, up to one update per resident workgroup, thetas up
  * to 64 x 64 or 128 x 32 factorised in LDS: many independent small updates, as in batches
  * of small instances). groups caps the

@@ -2,7 +2,7 @@
 
 `tessera_tebd_batch_run_ex` runs the updates of many MPS in super-steps of one
 cooperative launch, in one of two layouts: grouped (each update's SVD spread
-over a group of workgroups that advance in lockstep; few concurrent updates,
+over a group of workgroups that synchronise on the group's own barrier; few concurrent updates,
 large thetas) or solo (one workgroup per update, thetas up to 64 x 64 or 128 x 32 in LDS;
 up to one update per resident workgroup). Which is faster depends on the bond
 dimension and on how many independent updates the batch offers, so the tuner

@@ -431,6 +431,8 @@ __device__ __forceinline__ void lq_apply(double2* lds, int n, int i0, const doub
     const int rr = threadIdx.x & 15, g = threadIdx.x >> 4, ng = blockDim.x >> 4;
     const int j0 = i0 + ((g - i0 % ng + ng) % ng);  // first jj >= i0 with jj = g (mod ng)
     double2 w = make_double2(0, 0);
+    // unrolled so that several reflector loads are in flight (the additions keep their order)
+#pragma unroll 2
     for (int jj = j0; jj < n; jj += ng) w = cadd(w, cmul(lds[jj * 16 + rr], v[(std::size_t)jj * vs]));
     w = lq_row_sum(w, sh);
     if (rr > rmin) {

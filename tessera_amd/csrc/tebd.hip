@@ -1515,7 +1515,6 @@ struct ExecArgs {
     double2* aux;       // [G][ld * ld] LQ reflectors
     double2* taus;      // [G][ld]
     int small;          // solo layout: thetas within kSmall x kSmall take svd_small (dynamic LDS fits it)
-    int group_sync;     // grouped: each group's SVD on its own barrier (else the grid barrier, in lockstep)
     int* group_bar;     // [G][2] arrival count, generation
     int* err;           // raised by a GroupSync wait that gave up
     long long spin_limit;  // GroupSync polls before giving up
@@ -2367,7 +2366,6 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.snap_dims = t0.ex_sdims_.p;
     x.aux = t0.ex_aux_.p;
     x.taus = t0.ex_taus_.p;
-    x.group_sync = group_sync ? 1 : 0;
     x.err = t0.ex_bar_.p;  // the error word, then [G][2] barrier words
     x.spin_limit = t0.ex_spin_limit_;
     x.dataflow = (group_sync && t0.ex_dataflow_ && snap_list.empty()) ? 1 : 0;

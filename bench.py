@@ -277,7 +277,7 @@ def config1_bench(args):
 
 
 def batch_bench(args):
-    """Config-4 instances (n ~ U{16..96}, chi in {32, 64, 128}, banded sparse
+    """Batched-workload instances (n ~ U{16..96}, chi in {32, 64, 128}, banded sparse
     graphs), each from a saturated state (bonds at capacity, bond weights
     exp(-k/40)), advanced by `steps` mid-anneal Trotter steps: all together in
     one batched launch vs one at a time on the per-update path."""

@@ -56,6 +56,10 @@ struct Chain {
     // 1: accurate one-sided Jacobi SVD with the same keep rule; this is the
     //    upstream update in exact arithmetic, the definition the GPU follows.
     int svd_mode = 0;
+    // Updates whose kept rank sits at the rounding level of theta: the rank kept
+    // with a cut of 1e-13 differs from the one with 1e-15 (diagnostic only; the
+    // keep rule itself is the upstream 1e-14, mps.cpp:82).
+    long long ambiguous = 0;
     explicit Chain(int n);
     int n() const { return (int)site.size(); }
     int max_bond() const;
@@ -77,6 +81,7 @@ struct TebdResult {
     double energy = 0, expected_energy = 0, prob = 0, max_trunc = 0;
     std::vector<double> entropy;
     std::vector<int> max_bond;   // after every step (observability only)
+    std::vector<int> ambiguous;  // updates of each step with a rank at the rounding level
     std::vector<double> norm2;   // after every step, only when requested
     std::vector<cplx> state;     // dense final state, only when requested
 };

@@ -31,10 +31,14 @@ double oracle_qa_anneal(const double* h, std::size_t n, const std::size_t* ei, c
 // Optional (nullable): per-step entropy, max bond, norm2, and the dense final
 // state (2 * 2^n doubles, interleaved re/im; n <= 24). svd_mode 0 is the
 // upstream Gram SVD (bit-exact), 1 the accurate one-sided Jacobi SVD.
+// out_ambiguous[k] (nullable): updates of step k whose kept rank differs
+// between cuts of 1e-13 and 1e-15, i.e. singular values at the rounding level
+// of theta next to the 1e-14 cut.
 int oracle_tebd(const double* h, std::size_t n, const std::size_t* ei, const std::size_t* ej,
                 const double* ew, std::size_t m, const double* sa, const double* sb, std::size_t steps,
                 double dt, std::size_t chi, int zz_substeps, int8_t* out_s, double* out_scalars,
-                double* out_entropy, int* out_max_bond, double* out_norm2, double* out_state, int svd_mode) {
+                double* out_entropy, int* out_max_bond, double* out_norm2, double* out_state, int svd_mode,
+                int* out_ambiguous) {
     if (!h || !out_s || !out_scalars || n == 0) return -1;
     if (out_state && n > 24) return -2;
     std::vector<double> hv(h, h + n), ewv(ew, ew + m), av(sa, sa + steps), bv(sb, sb + steps);
@@ -54,6 +58,7 @@ int oracle_tebd(const double* h, std::size_t n, const std::size_t* ei, const std
         if (out_entropy) out_entropy[k] = r.entropy[k];
         if (out_max_bond) out_max_bond[k] = r.max_bond[k];
         if (out_norm2) out_norm2[k] = r.norm2[k];
+        if (out_ambiguous) out_ambiguous[k] = r.ambiguous[k];
     }
     if (out_state)
         for (std::size_t x = 0; x < r.state.size(); ++x) {

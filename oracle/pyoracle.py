@@ -35,7 +35,7 @@ def lib():
         L.oracle_qa_anneal.argtypes = [vp, sz, vp, vp, vp, sz, vp, vp, sz, sz, vp, vp]
         L.oracle_tebd.restype = ctypes.c_int
         L.oracle_tebd.argtypes = [vp, sz, vp, vp, vp, sz, vp, vp, sz, ctypes.c_double, sz, ctypes.c_int, vp, vp, vp,
-                                  vp, vp, vp, ctypes.c_int]
+                                  vp, vp, vp, ctypes.c_int, vp]
         L.oracle_sv.restype = ctypes.c_int
         L.oracle_sv.argtypes = [vp, sz, vp, vp, vp, sz, vp, vp, sz, ctypes.c_double, vp, vp, vp, vp, vp]
         L.oracle_herm_eig.restype = ctypes.c_int
@@ -64,24 +64,27 @@ def _edges(model):
 def tebd(model, sched, dt, chi, zz_substeps=1, svd_mode=0, want_norms=False, want_state=False):
     """Full TEBD run. svd_mode 0: upstream Gram SVD (bit-exact with the upstream
     engine); 1: accurate one-sided Jacobi SVD (the upstream update in exact
-    arithmetic)."""
+    arithmetic). `ambiguous[k]` counts the updates of step k whose kept rank is
+    set by singular values at the rounding level (cut 1e-13 and 1e-15 keep
+    different ranks)."""
     n, st = model.n, sched.steps
     ei, ej, ew = _edges(model)
     spins = np.zeros(n, np.int8)
     sc = np.zeros(4)
     ent = np.zeros(max(st, 1))
     mb = np.zeros(max(st, 1), np.int32)
+    amb = np.zeros(max(st, 1), np.int32)
     nr = np.zeros(max(st, 1)) if want_norms else None
     state = np.zeros(2 << n) if want_state else None
     a = np.ascontiguousarray(sched.a, np.float64)
     b = np.ascontiguousarray(sched.b, np.float64)
     rc = lib().oracle_tebd(_p(np.ascontiguousarray(model.h, np.float64)), n, _p(ei), _p(ej), _p(ew), model.m, _p(a),
                            _p(b), st, dt, chi, zz_substeps, _p(spins), _p(sc), _p(ent), _p(mb), _p(nr), _p(state),
-                           svd_mode)
+                           svd_mode, _p(amb))
     if rc != 0:
         raise RuntimeError(f"oracle_tebd failed ({rc})")
     out = dict(spins=spins, energy=sc[0], expected_energy=sc[1], prob=sc[2], max_trunc=sc[3], entropy=ent[:st],
-               max_bond=mb[:st])
+               max_bond=mb[:st], ambiguous=amb[:st])
     if want_norms:
         out["norm2"] = nr[:st]
     if want_state:

@@ -340,6 +340,10 @@ void Chain::gate2(int i, const cplx g[4][4], int chi, double& trunc) {
     for (double x : sv) total += x * x;
     int keep = std::min(chi, kfull);
     while (keep > 1 && sv[(std::size_t)keep - 1] < 1e-14) --keep;
+    int keep_lo = std::min(chi, kfull), keep_hi = keep_lo;
+    while (keep_lo > 1 && sv[(std::size_t)keep_lo - 1] < 1e-13) --keep_lo;
+    while (keep_hi > 1 && sv[(std::size_t)keep_hi - 1] < 1e-15) --keep_hi;
+    if (keep_lo != keep_hi) ++ambiguous;
     double disc = 0.0;
     for (int t = keep; t < kfull; ++t) disc += sv[(std::size_t)t] * sv[(std::size_t)t];
     if (total > 0) trunc = std::max(trunc, disc / total);
@@ -602,6 +606,8 @@ TebdResult tebd_anneal(const std::vector<double>& h, const std::vector<int>& ei,
         half_layer(A, B);
         res.entropy[(std::size_t)k] = psi.entropy(0);
         res.max_bond.push_back(psi.max_bond());
+        res.ambiguous.push_back((int)psi.ambiguous);
+        psi.ambiguous = 0;
         if (want_norms) res.norm2.push_back(psi.norm2());
     }
 

@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <set>
 #include <string>
 
 #include "../../include/tessera_qa.h"
@@ -88,17 +89,20 @@ int tessera_qa_anneal_ex(const double* h, size_t n, const size_t* ei, const size
         if (!std::isfinite(dt)) throw Error(TESSERA_ERR_NONFINITE, "dt is not finite");
         if (bias && !finite_all(bias, n)) throw Error(TESSERA_ERR_NONFINITE, "non-finite bias");
         if (chi == 0) throw Error(TESSERA_ERR_ARG, "chi == 0");
+        if (steps > (size_t)INT32_MAX) throw Error(TESSERA_ERR_ARG, "too many steps");
         Tebd t(make_model(h, n, ei, ej, ew, m, bias), (int)chi, dt, zz_substeps);
+        // device-side traces sized once: the step loop only enqueues work
+        t.ensure_trace((int)steps);
+        if (out_max_bond) t.ensure_max_bond((int)steps);
         for (size_t k = 0; k < steps; ++k) {
             t.step(schedule_a[k], schedule_b[k], (int)k);
-            if (out_max_bond) {
-                const std::vector<int> d = t.bonds();
-                int mb = 1;
-                for (int i = 1; i <= (int)n; ++i) mb = std::max(mb, d[i]);
-                out_max_bond[k] = mb;
-            }
+            if (out_max_bond) t.record_max_bond((int)k);
         }
         t.sync();
+        if (out_max_bond && steps) {
+            const std::vector<int> mb = t.max_bonds((int)steps);
+            std::memcpy(out_max_bond, mb.data(), steps * sizeof(int32_t));
+        }
         if (out_entropy && steps) {
             const std::vector<double> ent = t.entropy((int)steps);
             std::memcpy(out_entropy, ent.data(), steps * sizeof(double));
@@ -473,20 +477,25 @@ int tessera_tebd_batch_run_ex(tessera_tebd* const* sessions, size_t count, const
         if (mode < 0 || mode > 2) throw Error(TESSERA_ERR_ARG, "mode must be 0, 1 or 2");
         if (!sessions || (count && (!a || !b || !steps))) throw Error(TESSERA_ERR_ARG, "null argument");
         if (groups < -1) throw Error(TESSERA_ERR_ARG, "groups must be >= -1");
+        // validate every entry before any session advances: an error leaves all
+        // sessions as they were
+        std::set<const Tebd*> seen;
+        for (size_t i = 0; i < count; ++i) {
+            if (!sessions[i]) throw Error(TESSERA_ERR_ARG, "null session");
+            if (steps[i] > (size_t)INT32_MAX) throw Error(TESSERA_ERR_ARG, "too many steps");
+            if (steps[i] && (!a[i] || !b[i])) throw Error(TESSERA_ERR_ARG, "null schedule");
+            for (size_t k = 0; k < steps[i]; ++k)
+                if (!std::isfinite(a[i][k]) || !std::isfinite(b[i][k])) throw Error(TESSERA_ERR_NONFINITE, "non-finite schedule");
+            if (!seen.insert(sessions[i]->eng.get()).second) throw Error(TESSERA_ERR_ARG, "session listed twice");
+        }
         std::vector<Tebd*> ts;
         std::vector<const double*> as, bs;
         std::vector<int> cs;
         for (size_t i = 0; i < count; ++i) {
-            if (!sessions[i]) throw Error(TESSERA_ERR_ARG, "null session");
-            if (steps[i] && (!a[i] || !b[i])) throw Error(TESSERA_ERR_ARG, "null schedule");
-            for (size_t k = 0; k < steps[i]; ++k)
-                if (!std::isfinite(a[i][k]) || !std::isfinite(b[i][k])) throw Error(TESSERA_ERR_NONFINITE, "non-finite schedule");
             if (sessions[i]->eng->n() < 2) {  // nothing two-site to batch: plain path
                 sessions[i]->eng->run(a[i], b[i], (int)steps[i], -1);
                 continue;
             }
-            for (Tebd* t : ts)
-                if (t == sessions[i]->eng.get()) throw Error(TESSERA_ERR_ARG, "session listed twice");
             ts.push_back(sessions[i]->eng.get());
             as.push_back(a[i]);
             bs.push_back(b[i]);

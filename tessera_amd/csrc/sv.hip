@@ -110,7 +110,7 @@ __device__ inline int lay2(int t, int r) { return r | (t << 4); }               
 
 // ---------------------------------------------------------------------------
 // n <= 13: whole anneal in one workgroup, state resident in LDS.
-__global__ void __launch_bounds__(1024) k_sv_resident(Model md, const double* sa, const double* sb,
+__global__ void __launch_bounds__(1024) k_sv_resident(Model md, const double* sb,
                                                       const double* rot_c, const double* rot_s, double dt,
                                                       int steps, double amp0, double* ent_out,
                                                       double* en_out, double2* state_out) {
@@ -731,11 +731,11 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
     double total = 0.0, expected = 0.0, best_p = -1.0;
     long long best = 0;
     if (n <= kResidentMaxN) {
-        DevBuf<double> d_a(pb.steps), d_b(pb.steps), d_c(pb.steps), d_s(pb.steps);
+        DevBuf<double> d_b(pb.steps), d_c(pb.steps), d_s(pb.steps);
         TQ_HIP(hipMemcpy(d_b.p, pb.b.data(), pb.steps * sizeof(double), hipMemcpyHostToDevice));
         TQ_HIP(hipMemcpy(d_c.p, rc.data(), pb.steps * sizeof(double), hipMemcpyHostToDevice));
         TQ_HIP(hipMemcpy(d_s.p, rsn.data(), pb.steps * sizeof(double), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_sv_resident, dim3(1), dim3(1024), 0, st, md, d_a.p, d_b.p, d_c.p, d_s.p, pb.dt,
+        hipLaunchKernelGGL(k_sv_resident, dim3(1), dim3(1024), 0, st, md, d_b.p, d_c.p, d_s.p, pb.dt,
                            pb.steps, amp0, d_ent.p, d_en.p, psi.p);
         TQ_LAUNCH_CHECK();
         const int mblocks = (int)std::min<long long>((dim + 255) / 256, 2048);

@@ -1883,6 +1883,36 @@ void Tebd::ensure_trace(int count) {
     }
 }
 
+// Largest interior bond dims[1..n-1] (at least 1) into *out: one wavefront.
+__global__ void k_max_bond(const int* __restrict__ dims, int n, int* __restrict__ out) {
+    int m = 1;
+    for (int i = 1 + (int)threadIdx.x; i < n; i += 64) m = max(m, dims[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    if (threadIdx.x == 0) *out = m;
+}
+
+void Tebd::ensure_max_bond(int count) {
+    if (count > mbond_cap_) {
+        TQ_HIP(hipStreamSynchronize(stream_));
+        mbond_ = DevBuf<int>(count);
+        mbond_cap_ = count;
+    }
+}
+
+void Tebd::record_max_bond(int slot) {
+    if (slot < 0 || slot >= mbond_cap_) throw Error(-1, "max-bond slot out of range");
+    hipLaunchKernelGGL(k_max_bond, dim3(1), dim3(64), 0, stream_, dims_.p, n_, mbond_.p + slot);
+    TQ_LAUNCH_CHECK();
+}
+
+std::vector<int> Tebd::max_bonds(int count) {
+    if (count > mbond_cap_) throw Error(-1, "max-bond count out of range");
+    std::vector<int> out(count);
+    if (count) TQ_HIP(hipMemcpyAsync(out.data(), mbond_.p, count * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    sync();
+    return out;
+}
+
 void Tebd::reset_product(const double* warm) {
     std::vector<double2> host(off_[n_], make_double2(0, 0));
     const double amp = 1.0 / std::sqrt(2.0);

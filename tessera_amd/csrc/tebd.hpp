@@ -94,6 +94,12 @@ public:
     // Capacity of bond b (0..n): min(chi, 2^b, 2^(n-b)).
     int cap(int b) const { return cap_[b]; }
     void ensure_trace(int count);
+    // Largest bond after each step, kept on the device: size the buffer for
+    // `count` steps, record slot k after step k (enqueued, no host sync), read
+    // slots [0, count) after sync().
+    void ensure_max_bond(int count);
+    void record_max_bond(int slot);
+    std::vector<int> max_bonds(int count);
 
 private:
     void build_plan();
@@ -142,6 +148,8 @@ private:
     DevBuf<double2> envA_, envB_, envT_;
     DevBuf<double> trace_;
     int trace_cap_ = 0;
+    DevBuf<int> mbond_;
+    int mbond_cap_ = 0;
     std::size_t svd_lds_ = 0;
 
     // lockstep executor state

@@ -21,8 +21,10 @@ best plain energy. The structural template is python/gnn_guide.py:81-103
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import time
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -111,22 +113,45 @@ def sub_batch(batch: dict, g0: int, g1: int) -> dict:
 
 # ---------------------------------------------------------------- network
 
+# Networks still open at interpreter exit are destroyed by an atexit hook, before
+# the HIP runtime's own exit handlers run.
+_live_nets: "weakref.WeakSet[GuidanceNet]" = weakref.WeakSet()
+
+
+def _close_live_nets():
+    for g in list(_live_nets):
+        g.close()
+
+
+atexit.register(_close_live_nets)
+
+
 class GuidanceNet:
     """The device network: FP32 master weights, BF16 MFMA forward/backward, Adam."""
 
     def __init__(self, seed: int = 0):
         L = _bind()
         h = ctypes.c_void_p()
+        self._h = None
         _check(L.tessera_gnn_create(int(seed), ctypes.byref(h)))
         self._h = h
         self.count = int(L.tessera_gnn_param_count())
         self.graphs = 0
         self._batch = None
+        _live_nets.add(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def close(self):
         if self._h:
             _bind().tessera_gnn_destroy(self._h)
             self._h = None
+        _live_nets.discard(self)
 
     def __del__(self):
         try:

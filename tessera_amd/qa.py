@@ -13,9 +13,11 @@ raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import enum
 import math
+import weakref
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -268,23 +270,49 @@ def solve_qa(h, ei, ej, ew, steps: int, dt: float):
 
 
 # ---------------------------------------------------------- native TEBD session
+# Sessions still open when the interpreter exits are destroyed by an atexit hook,
+# which runs before module teardown and before the HIP runtime's own exit
+# handlers, so no stream, event or device buffer is released after the runtime
+# (or a profiler attached to it) has shut down.
+_live_sessions: "weakref.WeakSet[TebdSession]" = weakref.WeakSet()
+
+
+def _close_live_sessions():
+    for s in list(_live_sessions):
+        s.close()
+
+
+atexit.register(_close_live_sessions)
+
+
 class TebdSession:
-    """Device-resident MPS for repeated Trotter steps (tessera_tebd_* API)."""
+    """Device-resident MPS for repeated Trotter steps (tessera_tebd_* API).
+    Usable as a context manager; close() releases the device state."""
 
     def __init__(self, model: Ising, chi: int, dt: float, zz_substeps: int = 1):
         ei, ej, ew = _edges(model)
         h = np.ascontiguousarray(model.h, np.float64)
         handle = ctypes.c_void_p()
+        self._h = None
         _check(lib().tessera_tebd_create(_p(h), model.n, _p(ei), _p(ej), _p(ew), model.m, chi, dt, zz_substeps,
                                          ctypes.byref(handle)))
         self._h = handle
         self.n = model.n
         self.recorded = 0
+        _live_sessions.add(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def close(self):
         if self._h:
             lib().tessera_tebd_destroy(self._h)
             self._h = None
+        _live_sessions.discard(self)
 
     def __del__(self):
         try:
@@ -309,8 +337,10 @@ class TebdSession:
         self.recorded = 0
 
     def run(self, a, b, record: bool = True):
-        a = np.ascontiguousarray(a, np.float64)
-        b = np.ascontiguousarray(b, np.float64)
+        a = np.ascontiguousarray(a, np.float64).reshape(-1)
+        b = np.ascontiguousarray(b, np.float64).reshape(-1)
+        if a.shape != b.shape:
+            raise ValueError(f"schedule arrays differ in length: {a.shape[0]} vs {b.shape[0]}")
         _check(lib().tessera_tebd_run(self._h, _p(a), _p(b), a.shape[0], 1 if record else 0))
         if record:
             self.recorded += a.shape[0]
@@ -419,9 +449,15 @@ def batch_run(sessions, schedules, groups: int = -1, mode: int = 0):
     b_ptrs = (ctypes.c_void_p * k)()
     steps = np.zeros(k, np.uintp)
     handles = (ctypes.c_void_p * k)()
+    if len(schedules) != k:
+        raise ValueError(f"{k} sessions but {len(schedules)} schedules")
     for i, (s, sch) in enumerate(zip(sessions, schedules)):
-        a = np.ascontiguousarray(sch.a, np.float64)
-        b = np.ascontiguousarray(sch.b, np.float64)
+        a = np.ascontiguousarray(sch.a, np.float64).reshape(-1)
+        b = np.ascontiguousarray(sch.b, np.float64).reshape(-1)
+        if a.shape != b.shape:
+            raise ValueError(f"schedule {i}: a and b differ in length ({a.shape[0]} vs {b.shape[0]})")
+        if not s._h:
+            raise ValueError(f"session {i} is closed")
         keep += [a, b]
         a_ptrs[i] = a.ctypes.data
         b_ptrs[i] = b.ctypes.data

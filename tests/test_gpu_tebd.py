@@ -81,10 +81,15 @@ def test_exact_chi_matches_dense(n, chi, steps):
     g = _gpu_run(model, sched, 0.1, chi)
     psi, ents = dense_mps_splitting(model, sched, 0.1)
     inf = 1 - _fid(psi, g["state"])
-    print(f"n={n} chi={chi} infidelity={inf:.2e} ent_err={np.abs(g['entropy'] - ents).max():.2e}", flush=True)
+    # amplitude_prob (mps.cpp:197-216) of the readout: physical index 0 <-> +1 in bit i
+    x = sum(1 << i for i, s_ in enumerate(g["spins"]) if s_ < 0)
+    prob = abs(psi[x]) ** 2 / np.vdot(psi, psi).real
+    print(f"n={n} chi={chi} infidelity={inf:.2e} ent_err={np.abs(g['entropy'] - ents).max():.2e} "
+          f"prob={g['prob']:.12f} dense={prob:.12f}", flush=True)
     assert inf <= 1e-10
     assert np.abs(g["entropy"] - ents).max() <= 1e-9
     assert g["max_trunc"] <= 1e-20
+    assert abs(g["prob"] - prob) <= 1e-10
 
 
 def test_exact_chi_long_range_matches_dense():
@@ -115,6 +120,7 @@ def test_config1_prefix_matches_accurate_oracle():
     assert inf <= 1e-10
     assert np.abs(g["entropy"] - o["entropy"]).max() <= 1e-9
     assert abs(g["expected_energy"] - o["expected_energy"]) <= 1e-9
+    assert abs(g["prob"] - o["prob"]) <= 1e-10
     assert np.array_equal(g["spins"], o["spins"])
 
 
@@ -145,6 +151,7 @@ def test_truncated_ranks_and_weight_match_accurate_oracle():
     assert np.abs(ent - o["entropy"]).max() <= 1e-4
     assert np.array_equal(spins, o["spins"])
     assert abs(sc[1] - o["expected_energy"]) <= 1e-4
+    assert abs(sc[2] - o["prob"]) <= 1e-4 * max(o["prob"], 1e-3)
 
 
 def test_distributed_svd_path_matches(monkeypatch):

@@ -504,6 +504,138 @@ def gnn_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def gemm_flops_per_step(model, chi: int):
+    """Algorithmic flops of one Trotter step on a saturated MPS: per two-site
+    update on sites (p, p+1), F = 32 Dl Dr (mid + keep) real flops -- the theta
+    contraction (4 Dl mid Dr complex MACs, mps.cpp:49-70) plus the
+    reconstruction (4 Dl Dr keep complex MACs, linalg.cpp:133-139, mps.cpp:87-99)
+    at 8 flops per complex MAC -- with every bond at its capacity
+    cap(b) = min(chi, 2^b, 2^(n-b)) and keep = cap(p+1). The updates are those of
+    the swap-network routing of mps.cpp:126-145 (2r - 1 per edge of range r).
+    Returns (flops per step, updates per step)."""
+    n = model.n
+    cap = [min(chi, 2 ** min(b, 60), 2 ** min(n - b, 60)) for b in range(n + 1)]
+    flops, ups = 0.0, 0
+    for i, j in zip(model.ei, model.ej):
+        i, j = sorted((int(i), int(j)))
+        if i == j:
+            continue
+        sites = [k - 1 for k in range(j, i + 1, -1)] + [i] + list(range(i + 1, j))
+        for p in sites:
+            dl, mid, dr = cap[p], cap[p + 1], cap[p + 2]
+            flops += 32.0 * dl * dr * (mid + mid)
+            ups += 1
+    return flops, ups
+
+
+def pmc_traffic(kernel: str, updates: int):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (profiles/pmc_bytes.json: FETCH_SIZE and WRITE_SIZE per two-site update,
+    each scaled by the counter's reading for a plain copy of known size), or
+    None when no summary covers the kernel."""
+    f = ROOT / "profiles" / "pmc_bytes.json"
+    if not f.exists():
+        return None, None
+    rec = json.loads(f.read_text()).get(kernel)
+    if not rec:
+        return None, None
+    per = rec["fetch_bytes_per_update"] + rec["write_bytes_per_update"]
+    return per * updates, rec
+
+
+def tebd_timed(args, model, sched, k0, sess):
+    """The headline measurement on an open session (closed by the caller)."""
+    sess.set_executor(args.executor, args.exec_workers)
+    sess.randomize(SEED, 40.0)
+    ups = sess.updates_per_step
+    a = sched.a[k0:k0 + args.warmup + args.steps]
+    b = sched.b[k0:k0 + args.warmup + args.steps]
+    progress(f"n={args.n} chi={args.chi} updates/step={ups} executor={args.executor}")
+    if args.warmup:
+        tw = time.perf_counter()
+        sess.run(a[:args.warmup], b[:args.warmup], record=False)
+        sess.sync()
+        progress(f"warmup {args.warmup} step(s): {time.perf_counter() - tw:.2f}s")
+    sess.set_timing(True)
+    st0 = sess.svd_stats()
+    ex0 = sess.exec_stats()
+    pr0 = sess.exec_profile()
+    t0 = time.perf_counter()
+    # the timed steps are the anneal's own: single-site layers, routed ZZ block and
+    # the qubit-0 entropy of every step (recorded, as mps.cpp:355 does)
+    if args.executor:
+        sess.run(a[args.warmup:], b[args.warmup:], record=True)  # one launch: the steps overlap
+    else:
+        for k in range(args.steps):  # a progress line per step
+            sess.run(a[args.warmup + k:args.warmup + k + 1], b[args.warmup + k:args.warmup + k + 1], record=True)
+            sess.sync()
+            progress(f"step {k + 1}/{args.steps}: {time.perf_counter() - t0:.2f}s")
+    ent, trunc, bonds = sess.sync()
+    t1 = time.perf_counter()
+    progress(f"timed {args.steps} step(s): {t1 - t0:.2f}s")
+    st1 = sess.svd_stats()
+    wall = t1 - t0
+    updates = ups * args.steps
+    calls = st1["calls"] - st0["calls"]
+    vis = st1["visited_rows"] - st0["visited_rows"]
+    rot = st1["rotated_rows"] - st0["rotated_rows"]
+    kernel_ms = st1["svd_ms"]  # executor: the persistent kernel; else the SVD launches
+    jacobi_flops = 16.0 * vis + 18.0 * rot
+    f_step, ups_host = gemm_flops_per_step(model, args.chi)
+    caps = [min(args.chi, 2 ** min(x, 60), 2 ** min(args.n - x, 60)) for x in range(args.n + 1)]
+    saturated = [int(x) for x in bonds] == caps
+    gemm_flops = f_step * args.steps
+    achieved = gemm_flops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+    if args.executor:
+        ex1 = sess.exec_stats()
+        ss = ex1["supersteps"] - ex0["supersteps"]
+        kname = "k_tebd_lockstep"
+        kernel = ("k_tebd_lockstep (super-steps of theta, block-Jacobi SVDs on per-group barriers with "
+                  "point-to-point tournament rounds, reconstruction)")
+        pr1 = sess.exec_profile()
+        timing = {"kernel_ms": kernel_ms, "kernel_share_of_wall": kernel_ms * 1e-3 / wall, "supersteps": ss,
+                  "updates_per_superstep": (ex1["updates"] - ex0["updates"]) / max(ss, 1),
+                  "group_barriers": os.environ.get("TESSERA_EXEC_LOCKSTEP", "0") != "1",
+                  "point_to_point_rounds": (os.environ.get("TESSERA_EXEC_LOCKSTEP", "0") != "1"
+                                            and os.environ.get("TESSERA_EXEC_P2P", "1") != "0"),
+                  "workgroup0_phases": {k: round(pr1[k] - pr0[k], 3) for k in pr1}}
+    else:
+        kname = "k_svd_dist"
+        kernel = "per-update launches; timed: the SVD launches (k_svd_dist / k_svd)"
+        timing = {"svd_ms_per_update": kernel_ms / max(calls, 1), "svd_share_of_wall": kernel_ms * 1e-3 / wall}
+    sweeps = (st1["sweeps"] - st0["sweeps"]) / max(calls, 1)
+    traffic, pmc = pmc_traffic(f"{kname} n={args.n} chi={args.chi}", updates)
+    return {
+        "metric": "TEBD two-site updates/s @n=128 chi=256",
+        "value": updates / wall,
+        "unit": "two-site updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (fp64)",
+        "data": "synthetic: seeded range-4 chain, saturated MPS (random isometries, bond weights exp(-k/40))",
+        "config": {"workload": f"MPS-TEBD Trotter step n={args.n} chi={args.chi} range<=4 dt={DT}",
+                   "updates_per_step": ups, "max_bond": int(bonds.max()), "bonds_at_capacity": saturated,
+                   "svd_calls": calls, "jacobi_sweeps_per_svd": round(sweeps, 2),
+                   "max_trunc": trunc, "entropy_last": float(ent[-1]) if len(ent) else None,
+                   "executor_groups": args.executor, "executor_workers": args.exec_workers},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "basis": ("F = 32 Dl Dr (mid + keep) flops per update (theta + reconstruction GEMMs) summed "
+                               f"on the host over {args.steps} x {ups_host} updates of the saturated chain, "
+                               "/ the kernel's HIP-event time, / the FP64 peak (the arithmetic used)"),
+                     "flops_per_update_saturated": f_step / max(ups_host, 1),
+                     "jacobi_flops": jacobi_flops,
+                     "jacobi_tflops": jacobi_flops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0,
+                     "traffic_source": None if pmc is None else pmc.get("source"),
+                     "kernel": kernel, **timing},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -536,80 +668,8 @@ def main():
     model = I.chain_range(args.n, RANGE, SEED)
     sched = I.linear(SCHED_STEPS)
     k0 = SCHED_STEPS // 2
-    sess = qa.TebdSession(model, args.chi, DT)
-    sess.set_executor(args.executor, args.exec_workers)
-    sess.randomize(SEED, 40.0)
-    ups = sess.updates_per_step
-    a = sched.a[k0:k0 + args.warmup + args.steps]
-    b = sched.b[k0:k0 + args.warmup + args.steps]
-    progress(f"n={args.n} chi={args.chi} updates/step={ups} executor={args.executor}")
-    if args.warmup:
-        tw = time.perf_counter()
-        sess.run(a[:args.warmup], b[:args.warmup])
-        sess.sync()
-        progress(f"warmup {args.warmup} step(s): {time.perf_counter() - tw:.2f}s")
-    sess.set_timing(True)
-    st0 = sess.svd_stats()
-    ex0 = sess.exec_stats()
-    pr0 = sess.exec_profile()
-    t0 = time.perf_counter()
-    if args.executor:
-        sess.run(a[args.warmup:], b[args.warmup:])  # one launch: the steps overlap
-    else:
-        for k in range(args.steps):  # a progress line per step
-            sess.run(a[args.warmup + k:args.warmup + k + 1], b[args.warmup + k:args.warmup + k + 1])
-            sess.sync()
-            progress(f"step {k + 1}/{args.steps}: {time.perf_counter() - t0:.2f}s")
-    ent, trunc, bonds = sess.sync()
-    t1 = time.perf_counter()
-    progress(f"timed {args.steps} step(s): {t1 - t0:.2f}s")
-    st1 = sess.svd_stats()
-    wall = t1 - t0
-    updates = ups * args.steps
-    calls = st1["calls"] - st0["calls"]
-    vis = st1["visited_rows"] - st0["visited_rows"]
-    rot = st1["rotated_rows"] - st0["rotated_rows"]
-    svd_ms = st1["svd_ms"]  # executor: the persistent kernel; else the SVD launches
-    flops = 16.0 * vis + 18.0 * rot
-    achieved = flops / (svd_ms * 1e-3) / 1e12 if svd_ms > 0 else 0.0
-    if args.executor:
-        ex1 = sess.exec_stats()
-        ss = ex1["supersteps"] - ex0["supersteps"]
-        kernel = ("k_tebd_lockstep (super-steps of theta, block-Jacobi SVDs on per-group barriers with "
-                  "point-to-point tournament rounds, reconstruction; Jacobi flops counted)")
-        pr1 = sess.exec_profile()
-        timing = {"kernel_ms": svd_ms, "kernel_share_of_wall": svd_ms * 1e-3 / wall, "supersteps": ss,
-                  "updates_per_superstep": (ex1["updates"] - ex0["updates"]) / max(ss, 1),
-                  "group_barriers": os.environ.get("TESSERA_EXEC_LOCKSTEP", "0") != "1",
-                  "point_to_point_rounds": (os.environ.get("TESSERA_EXEC_LOCKSTEP", "0") != "1"
-                                            and os.environ.get("TESSERA_EXEC_P2P", "1") != "0"),
-                  "workgroup0_phases": {k: round(pr1[k] - pr0[k], 3) for k in pr1}}
-    else:
-        kernel = "one-sided Jacobi SVD (k_svd_dist / k_svd)"
-        timing = {"svd_ms_per_update": svd_ms / max(calls, 1), "svd_share_of_wall": svd_ms * 1e-3 / wall}
-    sweeps = (st1["sweeps"] - st0["sweeps"]) / max(calls, 1)
-
-    out = {
-        "metric": "TEBD two-site updates/s @n=128 chi=256",
-        "value": updates / wall,
-        "unit": "two-site updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "c128 (fp64)",
-        "data": "synthetic: seeded range-4 chain, saturated MPS (random isometries, bond weights exp(-k/40))",
-        "config": {"workload": f"MPS-TEBD Trotter step n={args.n} chi={args.chi} range<=4 dt={DT}",
-                   "updates_per_step": ups, "max_bond": int(bonds.max()),
-                   "svd_calls": calls, "jacobi_sweeps_per_svd": round(sweeps, 2),
-                   "max_trunc": trunc, "executor_groups": args.executor,
-                   "executor_workers": args.exec_workers},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": kernel, **timing},
-    }
+    with qa.TebdSession(model, args.chi, DT) as sess:
+        out = tebd_timed(args, model, sched, k0, sess)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.chi)
     print(json.dumps(out), flush=True)

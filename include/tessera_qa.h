@@ -95,6 +95,23 @@ int tessera_sv_anneal(const double* h, size_t n,
                       int8_t* out_s, double* out_prob, double* out_entropy, double* out_energy_trace,
                       double* out_energy, double* out_expected_energy);
 
+/*
+ * tessera_sv_anneal for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * upstream counterpart. Same arguments, results and errors as
+ * tessera_sv_anneal, but every device buffer of the anneal (state, schedule
+ * tables, traces, per-block partials) lives in one static 4 MiB array per GPU
+ * held until the process ends; the call performs no hipMalloc/hipFree.
+ * Limits: n <= 16, steps <= 16384, m <= 4096 (TESSERA_ERR_TOO_LARGE beyond).
+ * Calls on one GPU share the array: they are serialised inside a process, and
+ * each call overwrites what the previous one left there.
+ */
+int tessera_sv_anneal_static(const double* h, size_t n,
+                             const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                             const double* schedule_a, const double* schedule_b, size_t steps, double dt,
+                             int8_t* out_s, double* out_prob, double* out_entropy, double* out_energy_trace,
+                             double* out_energy, double* out_expected_energy);
+
 /* tessera_sv_anneal that also returns the final state (n <= 30 here, 2^n
  * complex amplitudes interleaved re/im in out_state, bit i of the index = spin i
  * set to +1): the amplitudes upstream quantum.rs:78-87 holds after the last step.

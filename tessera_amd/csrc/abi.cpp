@@ -137,15 +137,21 @@ double tessera_qa_anneal(const double* h, size_t n, const size_t* ei, const size
     return sc[0];
 }
 
-int tessera_sv_anneal(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew, size_t m,
-                      const double* schedule_a, const double* schedule_b, size_t steps, double dt, int8_t* out_s,
-                      double* out_prob, double* out_entropy, double* out_energy_trace, double* out_energy,
-                      double* out_expected_energy) {
+}  // extern "C"
+
+namespace {
+// tessera_sv_anneal and tessera_sv_anneal_static: the same anneal, device
+// buffers allocated per call or carved from the static scratch.
+int sv_entry(bool static_scratch, const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew,
+             size_t m, const double* schedule_a, const double* schedule_b, size_t steps, double dt, int8_t* out_s,
+             double* out_prob, double* out_entropy, double* out_energy_trace, double* out_energy,
+             double* out_expected_energy) {
     try {
         set_last_error("");
         check_model(h, n, ei, ej, ew, m, schedule_a, schedule_b, steps);
         if (!out_s) throw Error(TESSERA_ERR_ARG, "out_s is null");
         if (steps < 1) throw Error(TESSERA_ERR_ARG, "steps < 1");
+        if (steps > (size_t)INT32_MAX || m > (size_t)INT32_MAX) throw Error(TESSERA_ERR_ARG, "too many steps or edges");
         if (!std::isfinite(dt)) throw Error(TESSERA_ERR_NONFINITE, "dt is not finite");
         if (n > 36) throw Error(TESSERA_ERR_TOO_LARGE, "n > 36 does not fit one device");
         SvProblem pb;
@@ -162,7 +168,7 @@ int tessera_sv_anneal(const double* h, size_t n, const size_t* ei, const size_t*
         pb.b.assign(schedule_b, schedule_b + steps);
         pb.steps = (int)steps;
         pb.dt = dt;
-        const SvOutcome r = sv_anneal_device(pb, false);
+        const SvOutcome r = static_scratch ? sv_anneal_static(pb) : sv_anneal_device(pb, false);
         std::memcpy(out_s, r.spins.data(), n);
         if (out_prob) *out_prob = r.success_prob;
         if (out_entropy) std::memcpy(out_entropy, r.entropy.data(), steps * sizeof(double));
@@ -177,6 +183,25 @@ int tessera_sv_anneal(const double* h, size_t n, const size_t* ei, const size_t*
         set_last_error("unknown exception");
         return TESSERA_ERR_DEVICE;
     }
+}
+}  // namespace
+
+extern "C" {
+
+int tessera_sv_anneal(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew, size_t m,
+                      const double* schedule_a, const double* schedule_b, size_t steps, double dt, int8_t* out_s,
+                      double* out_prob, double* out_entropy, double* out_energy_trace, double* out_energy,
+                      double* out_expected_energy) {
+    return sv_entry(false, h, n, ei, ej, ew, m, schedule_a, schedule_b, steps, dt, out_s, out_prob, out_entropy,
+                    out_energy_trace, out_energy, out_expected_energy);
+}
+
+int tessera_sv_anneal_static(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew,
+                             size_t m, const double* schedule_a, const double* schedule_b, size_t steps, double dt,
+                             int8_t* out_s, double* out_prob, double* out_entropy, double* out_energy_trace,
+                             double* out_energy, double* out_expected_energy) {
+    return sv_entry(true, h, n, ei, ej, ew, m, schedule_a, schedule_b, steps, dt, out_s, out_prob, out_entropy,
+                    out_energy_trace, out_energy, out_expected_energy);
 }
 
 int tessera_sv_anneal_state(const double* h, size_t n, const size_t* ei, const size_t* ej, const double* ew,

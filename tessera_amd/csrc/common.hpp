@@ -63,6 +63,32 @@ struct DevBuf {
     }
 };
 
+// Device buffers of one call. Without a region each take() allocates its own
+// buffer, freed with the arena; with a region (base, cap) every take() carves
+// 256-byte aligned space from it and nothing is allocated (a request beyond
+// the region throws TESSERA_ERR_TOO_LARGE).
+struct DevArena {
+    unsigned char* base = nullptr;
+    std::size_t cap = 0, used = 0;
+    std::vector<DevBuf<unsigned char>> owned;
+    DevArena() = default;
+    DevArena(void* region, std::size_t bytes) : base(static_cast<unsigned char*>(region)), cap(bytes) {}
+    template <class T>
+    T* take(std::size_t count) {
+        const std::size_t bytes = count * sizeof(T);
+        if (!base) {
+            owned.emplace_back(bytes);
+            return reinterpret_cast<T*>(owned.back().p);
+        }
+        const std::size_t at = (used + 255) & ~(std::size_t)255;
+        if (at > cap || bytes > cap - at)
+            throw Error(-2, "static scratch of " + std::to_string(cap) + " bytes exceeded (" +
+                                std::to_string(at + bytes) + " needed)");
+        used = at + bytes;
+        return reinterpret_cast<T*>(base + at);
+    }
+};
+
 // ---- complex on double2 ---------------------------------------------------
 __host__ __device__ inline double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);

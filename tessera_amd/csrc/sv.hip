@@ -21,6 +21,7 @@
 // rounding (not bitwise).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "common.hpp"
 #include "sv.hpp"
@@ -694,7 +695,9 @@ __host__ __device__ double sv_entropy_from(double r00, double r11, double r01r, 
     return s;
 }
 
-SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
+SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state, DevArena* arena) {
+    DevArena own;
+    DevArena& ar = arena ? *arena : own;
     const int n = pb.n;
     if (n < 1 || n > 40) throw Error(-1, "state-vector anneal needs 1 <= n <= 40");
     if (pb.steps < 1) throw Error(-1, "state-vector anneal needs steps >= 1");
@@ -705,15 +708,17 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
     out.energy_trace.assign(pb.steps, 0.0);
 
     // model upload
-    DevBuf<int> d_ei(std::max(m, 1)), d_ej(std::max(m, 1));
-    DevBuf<double> d_ew(std::max(m, 1)), d_h(n);
+    int* d_ei = ar.take<int>(std::max(m, 1));
+    int* d_ej = ar.take<int>(std::max(m, 1));
+    double* d_ew = ar.take<double>(std::max(m, 1));
+    double* d_h = ar.take<double>(n);
     if (m) {
-        TQ_HIP(hipMemcpy(d_ei.p, pb.ei.data(), m * sizeof(int), hipMemcpyHostToDevice));
-        TQ_HIP(hipMemcpy(d_ej.p, pb.ej.data(), m * sizeof(int), hipMemcpyHostToDevice));
-        TQ_HIP(hipMemcpy(d_ew.p, pb.ew.data(), m * sizeof(double), hipMemcpyHostToDevice));
+        TQ_HIP(hipMemcpy(d_ei, pb.ei.data(), m * sizeof(int), hipMemcpyHostToDevice));
+        TQ_HIP(hipMemcpy(d_ej, pb.ej.data(), m * sizeof(int), hipMemcpyHostToDevice));
+        TQ_HIP(hipMemcpy(d_ew, pb.ew.data(), m * sizeof(double), hipMemcpyHostToDevice));
     }
-    TQ_HIP(hipMemcpy(d_h.p, pb.h.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    Model md{n, m, d_ei.p, d_ej.p, d_ew.p, d_h.p};
+    TQ_HIP(hipMemcpy(d_h, pb.h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    Model md{n, m, d_ei, d_ej, d_ew, d_h};
 
     // per-step rotation cos/sin from the host libm, as quantum.rs:191
     std::vector<double> rc(pb.steps), rsn(pb.steps);
@@ -724,30 +729,33 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
     }
     const double amp0 = 1.0 / std::sqrt((double)dim);  // quantum.rs:72
 
-    DevBuf<double2> psi(dim);
-    DevBuf<double> d_ent(pb.steps), d_en(pb.steps);
+    double2* psi = ar.take<double2>((std::size_t)dim);
+    double* d_ent = ar.take<double>(pb.steps);
+    double* d_en = ar.take<double>(pb.steps);
     hipStream_t st = nullptr;
 
     double total = 0.0, expected = 0.0, best_p = -1.0;
     long long best = 0;
     if (n <= kResidentMaxN) {
-        DevBuf<double> d_b(pb.steps), d_c(pb.steps), d_s(pb.steps);
-        TQ_HIP(hipMemcpy(d_b.p, pb.b.data(), pb.steps * sizeof(double), hipMemcpyHostToDevice));
-        TQ_HIP(hipMemcpy(d_c.p, rc.data(), pb.steps * sizeof(double), hipMemcpyHostToDevice));
-        TQ_HIP(hipMemcpy(d_s.p, rsn.data(), pb.steps * sizeof(double), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_sv_resident, dim3(1), dim3(1024), 0, st, md, d_b.p, d_c.p, d_s.p, pb.dt,
-                           pb.steps, amp0, d_ent.p, d_en.p, psi.p);
+        double* d_b = ar.take<double>(pb.steps);
+        double* d_c = ar.take<double>(pb.steps);
+        double* d_s = ar.take<double>(pb.steps);
+        TQ_HIP(hipMemcpy(d_b, pb.b.data(), pb.steps * sizeof(double), hipMemcpyHostToDevice));
+        TQ_HIP(hipMemcpy(d_c, rc.data(), pb.steps * sizeof(double), hipMemcpyHostToDevice));
+        TQ_HIP(hipMemcpy(d_s, rsn.data(), pb.steps * sizeof(double), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_sv_resident, dim3(1), dim3(1024), 0, st, md, d_b, d_c, d_s, pb.dt,
+                           pb.steps, amp0, d_ent, d_en, psi);
         TQ_LAUNCH_CHECK();
         const int mblocks = (int)std::min<long long>((dim + 255) / 256, 2048);
-        DevBuf<double> ps(mblocks * 3);
-        DevBuf<long long> pi(mblocks);
-        hipLaunchKernelGGL(k_sv_measure, dim3(mblocks), dim3(256), 0, st, psi.p, md, dim, ps.p, pi.p);
+        double* ps = ar.take<double>((std::size_t)mblocks * 3);
+        long long* pi = ar.take<long long>(mblocks);
+        hipLaunchKernelGGL(k_sv_measure, dim3(mblocks), dim3(256), 0, st, psi, md, dim, ps, pi);
         TQ_LAUNCH_CHECK();
         TQ_HIP(hipStreamSynchronize(st));
         std::vector<double> hs(mblocks * 3);
         std::vector<long long> hi(mblocks);
-        TQ_HIP(hipMemcpy(hs.data(), ps.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
-        TQ_HIP(hipMemcpy(hi.data(), pi.p, hi.size() * sizeof(long long), hipMemcpyDeviceToHost));
+        TQ_HIP(hipMemcpy(hs.data(), ps, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        TQ_HIP(hipMemcpy(hi.data(), pi, hi.size() * sizeof(long long), hipMemcpyDeviceToHost));
         for (int b = 0; b < mblocks; ++b) {
             total += hs[b * 3 + 0];
             expected += hs[b * 3 + 1];
@@ -760,10 +768,10 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
         const long long ntiles = dim >> kTileBits;
         const int nblocks = (int)std::min<long long>(ntiles, 4096);
         const bool fast_phase = true;  // register phase block wherever the L-H couplings fit
-        DevBuf<double> partials((std::size_t)nblocks * kObs);
-        DevBuf<double> bp(nblocks);
-        DevBuf<long long> bx(nblocks);
-        hipLaunchKernelGGL(k_sv_fill, dim3(2048), dim3(256), 0, st, psi.p, dim, amp0);
+        double* partials = ar.take<double>((std::size_t)nblocks * kObs);
+        double* bp = ar.take<double>(nblocks);
+        long long* bx = ar.take<long long>(nblocks);
+        hipLaunchKernelGGL(k_sv_fill, dim3(2048), dim3(256), 0, st, psi, dim, amp0);
         TQ_LAUNCH_CHECK();
         // rotation groups: bits [0,12) as contiguous blocks, then windows of <= 9 bits
         struct Group {
@@ -826,17 +834,17 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
             pts[g].off[kKinds] = acc;
             if (acc > kMaxTerms) throw Error(-1, "more than 256 energy terms (m + n) per tile");
         }
-        DevBuf<Term> d_terms(std::max<std::size_t>(terms.size(), 1));
+        Term* d_terms = ar.take<Term>(std::max<std::size_t>(terms.size(), 1));
         if (!terms.empty())
-            TQ_HIP(hipMemcpy(d_terms.p, terms.data(), terms.size() * sizeof(Term), hipMemcpyHostToDevice));
-        for (int g = 0; g < ng; ++g) pts[g].t = d_terms.p + starts[g];
+            TQ_HIP(hipMemcpy(d_terms, terms.data(), terms.size() * sizeof(Term), hipMemcpyHostToDevice));
+        for (int g = 0; g < ng; ++g) pts[g].t = d_terms + starts[g];
 
         auto launch = [&](int g, const PassArgs& pa, bool ph) {
             // the passes the host loop below issues: rotate only; phase then rotate (first);
             // rotate, phase, rotate (step boundary); rotate then phase (last)
             auto go = [&](auto kern) {
-                hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kThreads), 0, st, psi.p, pts[g], pa, ntiles, partials.p,
-                                   bp.p, bx.p);
+                hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kThreads), 0, st, psi, pts[g], pa, ntiles, partials,
+                                   bp, bx);
             };
             const bool f = pa.fast != 0;
             if (!ph && pa.rot_a && !pa.rot_b) go(k_sv_pass<false, true, false, false>);
@@ -900,8 +908,8 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
                         pa.best = 1;
                     }
                     launch(g, pa, true);
-                    hipLaunchKernelGGL(k_sv_finish_step, dim3(1), dim3(256), 0, st, partials.p, nblocks, n, k,
-                                       d_ent.p, d_en.p);
+                    hipLaunchKernelGGL(k_sv_finish_step, dim3(1), dim3(256), 0, st, partials, nblocks, n, k,
+                                       d_ent, d_en);
                     TQ_LAUNCH_CHECK();
                 }
             }
@@ -909,9 +917,9 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
         TQ_HIP(hipStreamSynchronize(st));
         std::vector<double> hs((std::size_t)nblocks * kObs), hbp(nblocks);
         std::vector<long long> hbx(nblocks);
-        TQ_HIP(hipMemcpy(hs.data(), partials.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
-        TQ_HIP(hipMemcpy(hbp.data(), bp.p, nblocks * sizeof(double), hipMemcpyDeviceToHost));
-        TQ_HIP(hipMemcpy(hbx.data(), bx.p, nblocks * sizeof(long long), hipMemcpyDeviceToHost));
+        TQ_HIP(hipMemcpy(hs.data(), partials, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        TQ_HIP(hipMemcpy(hbp.data(), bp, nblocks * sizeof(double), hipMemcpyDeviceToHost));
+        TQ_HIP(hipMemcpy(hbx.data(), bx, nblocks * sizeof(long long), hipMemcpyDeviceToHost));
         for (int b = 0; b < nblocks; ++b) {
             total += hs[(std::size_t)b * kObs + 4];
             expected += hs[(std::size_t)b * kObs + 5];
@@ -922,8 +930,8 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
         }
     }
     TQ_HIP(hipStreamSynchronize(st));
-    TQ_HIP(hipMemcpy(out.entropy.data(), d_ent.p, pb.steps * sizeof(double), hipMemcpyDeviceToHost));
-    TQ_HIP(hipMemcpy(out.energy_trace.data(), d_en.p, pb.steps * sizeof(double), hipMemcpyDeviceToHost));
+    TQ_HIP(hipMemcpy(out.entropy.data(), d_ent, pb.steps * sizeof(double), hipMemcpyDeviceToHost));
+    TQ_HIP(hipMemcpy(out.energy_trace.data(), d_en, pb.steps * sizeof(double), hipMemcpyDeviceToHost));
     out.spins.resize(n);
     for (int i = 0; i < n; ++i) out.spins[i] = ((best >> i) & 1) ? 1 : -1;
     // energy of the argmax in the upstream order (quantum.rs:106)
@@ -936,9 +944,49 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state) {
     out.best_index = best;
     if (keep_state) {
         out.state.resize(2 * (std::size_t)dim);
-        TQ_HIP(hipMemcpy(out.state.data(), psi.p, dim * sizeof(double2), hipMemcpyDeviceToHost));
+        TQ_HIP(hipMemcpy(out.state.data(), psi, dim * sizeof(double2), hipMemcpyDeviceToHost));
     }
     return out;
+}
+
+
+// ---- static scratch: tessera_sv_anneal without device allocation ----------
+// One array per GPU, held until the process ends; every call on a GPU reuses
+// it (nothing in it needs to start at zero: each call writes every slot it
+// reads). Host code never passes the array's name to a kernel or a copy: its
+// address on each device comes from hipGetSymbolAddress, taken once per device
+// with that device current. Calls on one GPU are serialised by a per-device
+// mutex inside this process.
+static __device__ __align__(256) unsigned char scratch[kSvStaticBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu[kMaxDevices];
+void* g_static_ptr[kMaxDevices] = {};
+std::once_flag g_static_once[kMaxDevices];
+}  // namespace
+
+SvOutcome sv_anneal_static(const SvProblem& pb) {
+    if (pb.n < 1 || pb.n > kSvStaticMaxN)
+        throw Error(-2, "static-scratch anneal needs 1 <= n <= " + std::to_string(kSvStaticMaxN));
+    if (pb.steps < 1 || pb.steps > kSvStaticMaxSteps)
+        throw Error(-2, "static-scratch anneal needs 1 <= steps <= " + std::to_string(kSvStaticMaxSteps));
+    if (pb.ew.size() > (std::size_t)kSvStaticMaxEdges)
+        throw Error(-2, "static-scratch anneal takes at most " + std::to_string(kSvStaticMaxEdges) + " edges");
+    int dev = 0;
+    TQ_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(-3, "device index out of range");
+    hipError_t status = hipSuccess;
+    std::call_once(g_static_once[dev], [&] {
+        void* ptr = nullptr;
+        status = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (status == hipSuccess) g_static_ptr[dev] = ptr;
+    });
+    if (status != hipSuccess) throw Error(-3, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(status));
+    if (!g_static_ptr[dev]) throw Error(-3, "static scratch address unavailable on this device");
+    std::lock_guard<std::mutex> lock(g_static_mu[dev]);
+    DevArena ar(g_static_ptr[dev], kSvStaticBytes);
+    return sv_anneal_device(pb, false, &ar);
 }
 
 }  // namespace tq

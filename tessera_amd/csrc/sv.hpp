@@ -5,6 +5,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "common.hpp"
+
 namespace tq {
 
 struct SvProblem {
@@ -28,6 +30,17 @@ struct SvOutcome {  // quantum.rs:27-43
 // Qubit-0 entropy from the reduced density matrix sums (quantum.rs:156-174).
 __host__ __device__ double sv_entropy_from(double r00, double r11, double r01r, double r01i);
 
-SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state);
+// Device memory of one anneal (state, traces, tables, per-block partials) comes
+// from `arena`: nullptr allocates it for the call; a region-backed arena carves
+// it from caller-provided memory and allocates nothing.
+SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state, DevArena* arena = nullptr);
+
+// tessera_sv_anneal without device allocation: every buffer of the anneal lives
+// in one static device array of kSvStaticBytes per GPU (n <= kSvStaticMaxN).
+constexpr int kSvStaticMaxN = 16;
+constexpr int kSvStaticMaxSteps = 16384;
+constexpr int kSvStaticMaxEdges = 4096;
+constexpr std::size_t kSvStaticBytes = (std::size_t)4 << 20;
+SvOutcome sv_anneal_static(const SvProblem& pb);
 
 }  // namespace tq

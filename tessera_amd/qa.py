@@ -58,6 +58,9 @@ def lib() -> ctypes.CDLL:
         L.tessera_sv_anneal.restype = ctypes.c_int
         L.tessera_sv_anneal.argtypes = [vp, _usz, vp, vp, vp, _usz, vp, vp, _usz, ctypes.c_double, vp, vp, vp, vp,
                                         vp, vp]
+        L.tessera_sv_anneal_static.restype = ctypes.c_int
+        L.tessera_sv_anneal_static.argtypes = [vp, _usz, vp, vp, vp, _usz, vp, vp, _usz, ctypes.c_double, vp, vp, vp,
+                                               vp, vp, vp]
         L.tessera_sv_anneal_state.restype = ctypes.c_int
         L.tessera_sv_anneal_state.argtypes = [vp, _usz, vp, vp, vp, _usz, vp, vp, _usz, ctypes.c_double, vp, vp, vp,
                                               vp]
@@ -231,6 +234,26 @@ def anneal_statevector(model: Ising, sched: Schedule, dt: float) -> QaOutcome:
     _check(lib().tessera_sv_anneal(_p(model.h), model.n, _p(ei), _p(ej), _p(ew), model.m, _p(a), _p(b), steps, dt,
                                    _p(spins), ctypes.addressof(prob), _p(ent), _p(etr), ctypes.addressof(en),
                                    ctypes.addressof(ex)))
+    return QaOutcome(spins, en.value, ex.value, prob.value, ent, etr)
+
+
+def anneal_statevector_static(model: Ising, sched: Schedule, dt: float) -> QaOutcome:
+    """anneal_statevector through tessera_sv_anneal_static: no device allocation
+    (every buffer in the library's static per-GPU scratch; n <= 16, steps <= 16384,
+    m <= 4096)."""
+    ei, ej, ew = _edges(model)
+    steps = sched.steps
+    spins = np.zeros(model.n, np.int8)
+    prob = ctypes.c_double()
+    en = ctypes.c_double()
+    ex = ctypes.c_double()
+    ent = np.zeros(steps)
+    etr = np.zeros(steps)
+    a = np.ascontiguousarray(sched.a, np.float64)
+    b = np.ascontiguousarray(sched.b, np.float64)
+    _check(lib().tessera_sv_anneal_static(_p(model.h), model.n, _p(ei), _p(ej), _p(ew), model.m, _p(a), _p(b), steps,
+                                          dt, _p(spins), ctypes.addressof(prob), _p(ent), _p(etr),
+                                          ctypes.addressof(en), ctypes.addressof(ex)))
     return QaOutcome(spins, en.value, ex.value, prob.value, ent, etr)
 
 

@@ -151,3 +151,21 @@ def test_guidance_and_hook_entries_reject_bad_arguments():
     st = np.zeros(8)
     assert Lq.tessera_sv_anneal_state(qa._p(h), 2, None, None, None, 0, None, None, 0, 0.1, qa._p(spins), None, None,
                                       qa._p(st)) == -1  # steps < 1
+
+
+def test_static_scratch_entry_rejects_sizes_beyond_its_limits():
+    """tessera_sv_anneal_static checks n, steps and m against its static scratch
+    before any device work."""
+    L = qa.lib()
+    sp = np.zeros(17, np.int8)
+    for n, steps, m in ((17, 4, 0), (4, 16385, 0), (4, 4, 4097)):
+        h = np.zeros(n)
+        ei = np.zeros(m, np.uintp)
+        ej = np.ones(m, np.uintp)
+        ew = np.ones(m)
+        a = np.ones(steps)
+        b = np.ones(steps)
+        rc = L.tessera_sv_anneal_static(*_args(h, ei, ej, ew, a, b), 0.1, qa._p(sp), None, None, None, None, None)
+        assert rc == -2, (n, steps, m)
+        assert b"static-scratch" in L.tessera_last_error()
+

@@ -163,6 +163,7 @@ private:
     DevBuf<int> ex_bar_;                   // the barrier error word, then group barriers [G][2]
     DevBuf<unsigned long long> ex_prof_;   // phase profile of workgroup 0
     bool ex_group_sync_ = true;            // grouped layout: per-group barriers inside an update
+    bool ex_xcd_local_ = false;            // ... XCD-local when the groups sit one per XCD (opt-in)
     bool ex_armed_ = false;                // a launch whose barrier error word is still unread
     long long ex_spin_limit_ = 1ll << 24;  // GroupSync polls before a wait gives up
     bool ex_p2p_ = true;                   // per-group layout: point-to-point tournament rounds

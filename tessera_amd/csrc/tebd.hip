@@ -28,6 +28,7 @@
 
 #include <hip/hip_cooperative_groups.h>
 
+#include "mfma64.hpp"
 #include "tebd.hpp"
 
 namespace tq {
@@ -109,48 +110,96 @@ __global__ void __launch_bounds__(256) k_single_layer(double2* sites, const std:
     }
 }
 
-// ---- theta (mps.cpp:49-70) ------------------------------------------------
+// ---- theta (mps.cpp:49-70) on the matrix cores ------------------------------
+// P = A C with A = site p as a (2 Dl x mid) matrix (rows (a, s1), row stride
+// cap[p+1]) and C = site p+1 as a (mid x 2 Dr) matrix (columns (s2, b), row
+// stride 2 cap[p+2]): complex FP64 MFMA products, K = mid in steps of 4. A
+// wavefront's tile covers a = a0..a0+7 (both s1) and b = b0..b0+15 in two
+// 16 x 16 accumulators (s2 = 0, 1); its tile rows are ordered so that every lane
+// ends up holding all four P[(a, s1), (s2, b)] of two values of a, and the 4x4
+// gate is applied in registers: theta[(a,r1),(r2,b)] = sum G[r1r2][s1s2] P.
+// theta is written column-major as M = theta (2 Dr <= 2 Dl) or theta^dagger,
+// and copied to W (the SVD's working matrix). Tiles are dealt to waves
+// `wave`, `wave + nwaves`, ...
+__device__ void theta_mfma(const double2* __restrict__ sites, const std::size_t* off, const int* cap,
+                           const int* dims, int p, const Gate4& g, double2* __restrict__ M,
+                           double2* __restrict__ W, int ld, long long wave, long long nwaves) {
+    const int Dl = dims[p], mid = dims[p + 1], Dr = dims[p + 2];
+    const int sA = cap[p + 1], sC = cap[p + 2];
+    const double2* A = sites + off[p];
+    const double2* C = sites + off[p + 1];
+    const int lane = (int)(threadIdx.x & 63), lg = lane >> 4, lc = lane & 15;
+    const int nta = (Dl + 7) / 8, ntb = (Dr + 15) / 16;
+    const bool tall = Dr <= Dl;
+    const double2 z = make_double2(0, 0);
+    // the gate in LDS: the epilogue reads its coefficients where it needs them
+    // (kept in registers across the tile loop they would crowd out the products)
+    __shared__ double2 gl[16];
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) gl[q] = g.g[q];
+    __syncthreads();
+    for (long long t = wave; t < (long long)nta * ntb; t += nwaves) {
+        const int a0 = (int)(t / ntb) * 8, b0 = (int)(t % ntb) * 16;
+        // operand row lc = tile row r <-> (a, s1) with r = g + 4 i, a = a0 + 2 g + i / 2, s1 = i % 2
+        const int ra = a0 + 2 * (lc & 3) + (lc >> 3), rs = (lc >> 2) & 1;
+        const int bb = b0 + lc;
+        const bool arow = ra < Dl, bcol = bb < Dr;
+        const double2* ap = A + (std::size_t)(ra * 2 + rs) * sA;
+        CTile t0, t1;
+        t0.zero();
+        t1.zero();
+        for (int c0 = 0; c0 < mid; c0 += 8) {
+            double2 av[2], b0v[2], b1v[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {  // two K steps of loads in flight
+                const int c = c0 + 4 * u + lg;
+                const bool ok = c < mid;
+                av[u] = (ok && arow) ? ap[c] : z;
+                b0v[u] = (ok && bcol) ? C[(std::size_t)(c * 2 + 0) * sC + bb] : z;
+                b1v[u] = (ok && bcol) ? C[(std::size_t)(c * 2 + 1) * sC + bb] : z;
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                t0.mac(av[u], b0v[u]);
+                t1.mac(av[u], b1v[u]);
+            }
+        }
+        if (!bcol) continue;
+        // elements 2h, 2h+1 hold a = a0 + 2 lg + h with s1 = 0, 1
+        auto emit = [&](int a, double2 p00, double2 p01, double2 p10, double2 p11) {  // p<s1><s2>
+            if (a >= Dl) return;
+#pragma unroll
+            for (int r1 = 0; r1 < 2; ++r1)
+#pragma unroll
+                for (int r2 = 0; r2 < 2; ++r2) {
+                    asm volatile("" ::: "memory");  // re-read the coefficients here
+                    const double2* gq = gl + (r1 * 2 + r2) * 4;
+                    const double2 acc =
+                        cadd(cadd(cmul(gq[0], p00), cmul(gq[1], p01)), cadd(cmul(gq[2], p10), cmul(gq[3], p11)));
+                    const int row = a * 2 + r1, col = r2 * Dr + bb;
+                    const std::size_t at = tall ? (std::size_t)col * ld + row : (std::size_t)row * ld + col;
+                    const double2 v = tall ? acc : cconj(acc);
+                    M[at] = v;
+                    W[at] = v;
+                }
+        };
+        emit(a0 + 2 * lg, t0.get(0), t1.get(0), t0.get(1), t1.get(1));
+        emit(a0 + 2 * lg + 1, t0.get(2), t1.get(2), t0.get(3), t1.get(3));
+    }
+}
+
+// Number of theta_mfma tiles for bond dimensions up to Dl x Dr.
+inline long long theta_tiles_for(int Dl, int Dr) { return (long long)((Dl + 7) / 8) * ((Dr + 15) / 16); }
+
+// theta of the per-update path: one tile per wavefront, grid sized for the
+// capacities of the bonds (tiles outside the current dimensions do nothing).
 __global__ void __launch_bounds__(256) k_theta(const double2* __restrict__ sites, const std::size_t* off,
                                                const int* cap, const int* dims, int p, Gate4 g,
                                                double2* __restrict__ M, double2* __restrict__ W, int ld) {
-    __shared__ double2 As[16][2][17];
-    __shared__ double2 Cs[16][2][17];
-    const int Dl = dims[p], mid = dims[p + 1], Dr = dims[p + 2];
-    const int sA = cap[p + 1], sC = cap[p + 2];
-    const std::size_t oA = off[p], oC = off[p + 1];
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int a = blockIdx.y * 16 + ty, b = blockIdx.x * 16 + tx;
-    if (blockIdx.y * 16 >= Dl || blockIdx.x * 16 >= Dr) return;  // whole tile outside (uniform)
-    double2 th[2][2] = {{make_double2(0, 0), make_double2(0, 0)}, {make_double2(0, 0), make_double2(0, 0)}};
-    for (int c0 = 0; c0 < mid; c0 += 16) {
-        for (int s = 0; s < 2; ++s) {
-            const int aa = blockIdx.y * 16 + ty, cc = c0 + tx;
-            As[ty][s][tx] = (aa < Dl && cc < mid) ? ld_site(sites, oA, sA, aa, s, cc) : make_double2(0, 0);
-            const int cr = c0 + ty, bb = blockIdx.x * 16 + tx;
-            Cs[ty][s][tx] = (cr < mid && bb < Dr) ? ld_site(sites, oC, sC, cr, s, bb) : make_double2(0, 0);
-        }
-        __syncthreads();
-        for (int cc = 0; cc < 16; ++cc)
-            for (int s1 = 0; s1 < 2; ++s1) {
-                const double2 x = As[ty][s1][cc];
-                th[s1][0] = cadd(th[s1][0], cmul(x, Cs[cc][0][tx]));
-                th[s1][1] = cadd(th[s1][1], cmul(x, Cs[cc][1][tx]));
-            }
-        __syncthreads();
-    }
-    if (a >= Dl || b >= Dr) return;
-    const bool tall = Dr <= Dl;
-    for (int r1 = 0; r1 < 2; ++r1)
-        for (int r2 = 0; r2 < 2; ++r2) {
-            double2 acc = make_double2(0, 0);
-            for (int s1 = 0; s1 < 2; ++s1)
-                for (int s2 = 0; s2 < 2; ++s2) acc = cadd(acc, cmul(g.g[(r1 * 2 + r2) * 4 + s1 * 2 + s2], th[s1][s2]));
-            const int row = a * 2 + r1, col = r2 * Dr + b;
-            const std::size_t at = tall ? (std::size_t)col * ld + row : (std::size_t)row * ld + col;
-            const double2 v = tall ? acc : cconj(acc);
-            M[at] = v;
-            W[at] = v;
-        }
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    theta_mfma(sites, off, cap, dims, p, g, M, W, ld, wave, nwaves);
 }
 
 // ---- one-sided Jacobi SVD, one workgroup -----------------------------------
@@ -1166,7 +1215,6 @@ __device__ void recon_part(double2* sites, const std::size_t* off, const int* ca
     double2* A = sites + off[p];
     double2* Cn = sites + off[p + 1];
     const long long ndirect = tall ? (long long)R * keep : (long long)keep * C;
-    const long long ngemm = tall ? (long long)keep * C : (long long)R * keep;
     for (long long idx = gid; idx < ndirect; idx += stride) {
         if (tall) {
             const int row = (int)(idx / keep), k = (int)(idx % keep);
@@ -1181,37 +1229,61 @@ __device__ void recon_part(double2* sites, const std::size_t* off, const int* ca
             Cn[((std::size_t)k * 2 + s2) * sC + b] = wts ? (s > 0 ? cscale(v, wts[k] / s) : make_double2(0, 0)) : v;
         }
     }
-    // products, one output per wave: the lanes take every 64th term of the dot
-    // product (coalesced rows of both columns) and the wave sums them (gid and
-    // stride are thread indices in whole waves)
-    const int lane = (int)(gid & 63);
-    for (long long g = gid >> 6; g < ngemm; g += stride >> 6) {
-        double2 acc = make_double2(0, 0);
+    // products on the matrix cores, one 16 x 16 output tile per wave at a time
+    // (gid and stride are thread indices in whole waves):
+    //   tall: C'[k, col] = sum_row conj(W[perm k][row]) M[col][row] / S_k  (keep x C, K = R)
+    //   wide: A'[row, k] = sum_col conj(M[row][col]) W[perm k][col] / S_k^2 (R x keep, K = C)
+    const int lane = (int)(gid & 63), lg = lane >> 4, lc = lane & 15;
+    const long long wave = gid >> 6, nwaves = stride >> 6;
+    const double2 z = make_double2(0, 0);
+    const int rows_out = tall ? keep : R, cols_out = tall ? C : keep, K = tall ? R : C;
+    const int ntr = (rows_out + 15) / 16, ntc = (cols_out + 15) / 16;
+    for (long long t = wave; t < (long long)ntr * ntc; t += nwaves) {
+        const int r0 = (int)(t / ntc) * 16, c0 = (int)(t % ntc) * 16;
+        const int ra = r0 + lc, cb = c0 + lc;  // operand row of A, operand column of B for this lane
+        // tall: A row ra is singular vector k = ra (column perm k of W), B column cb is M column cb;
+        // wide: A row ra is M row ra, B column cb is singular vector k = cb
+        const double2* ap;
+        const double2* bp;
+        bool aok, bok;
         if (tall) {
-            const int k = (int)(g / C), col = (int)(g % C);
-            const double2* u = W + (std::size_t)gld<XL>(perm + k) * ld;
-            const double2* mc = M + (std::size_t)col * ld;
-#pragma unroll 2
-            for (int row = lane; row < R; row += 64) acc = cadd(acc, cmulc(gld<XL>(u + row), gld<XL>(mc + row)));
-            acc.x = wave_sum(acc.x);
-            acc.y = wave_sum(acc.y);
-            if (lane == 0) {
-                const double s = gld<XL>(sv + k);
-                const int s2 = col / Dr, b = col % Dr;
-                const double f = s > 0 ? (wts ? wts[k] / (s * s) : 1.0 / s) : 0.0;
-                Cn[((std::size_t)k * 2 + s2) * sC + b] = cscale(acc, f);
-            }
+            aok = ra < keep;
+            bok = cb < C;
+            ap = W + (std::size_t)(aok ? gld<XL>(perm + ra) : 0) * ld;
+            bp = M + (std::size_t)(bok ? cb : 0) * ld;
         } else {
-            const int row = (int)(g / keep), k = (int)(g % keep);
-            const double2* w = W + (std::size_t)gld<XL>(perm + k) * ld;
-            const double2* mr = M + (std::size_t)row * ld;
-#pragma unroll 2
-            for (int col = lane; col < C; col += 64) acc = cadd(acc, cmulc(gld<XL>(mr + col), gld<XL>(w + col)));
-            acc.x = wave_sum(acc.x);
-            acc.y = wave_sum(acc.y);
-            if (lane == 0) {
-                const double s = gld<XL>(sv + k);
-                A[(std::size_t)row * sA + k] = s > 0 ? cscale(acc, 1.0 / (s * s)) : make_double2(0, 0);
+            aok = ra < R;
+            bok = cb < keep;
+            ap = M + (std::size_t)(aok ? ra : 0) * ld;
+            bp = W + (std::size_t)(bok ? gld<XL>(perm + cb) : 0) * ld;
+        }
+        CTile acc;
+        acc.zero();
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            double2 av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {  // four K steps of loads in flight
+                const int kk = k0 + 4 * u + lg;
+                const bool ok = kk < K;
+                av[u] = (ok && aok) ? cconj(gld<XL>(ap + kk)) : z;
+                bv[u] = (ok && bok) ? gld<XL>(bp + kk) : z;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc.mac(av[u], bv[u]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = r0 + mfma_out_row(lg, i);
+            const double2 d = acc.get(i);
+            if (r >= rows_out || cb >= cols_out) continue;
+            if (tall) {  // r = k, cb = col
+                const double sk = gld<XL>(sv + r);
+                const double f = sk > 0 ? (wts ? wts[r] / (sk * sk) : 1.0 / sk) : 0.0;
+                const int s2 = cb / Dr, b = cb % Dr;
+                Cn[((std::size_t)r * 2 + s2) * sC + b] = cscale(d, f);
+            } else {  // r = row, cb = k
+                const double sk = gld<XL>(sv + cb);
+                A[(std::size_t)r * sA + cb] = sk > 0 ? cscale(d, 1.0 / (sk * sk)) : z;
             }
         }
     }
@@ -1453,58 +1525,14 @@ __global__ void k_randomize(double2* sites, const std::size_t* off, const int* c
     }
 }
 
-// theta of update p by 16x16 tiles (the computation of k_theta); each workgroup
-// computes blockDim/256 tiles at a time, tiles dealt round-robin over the group's
-// workgroups.
+// theta of update p (theta_mfma) by the waves of `nwg` workgroups, this one
+// being number `rank`.
 __device__ void theta_tiles(const double2* __restrict__ sites, const std::size_t* off, const int* cap,
                             const int* dims, int p, const Gate4& g, double2* __restrict__ M, double2* __restrict__ W,
-                            int ld, int rank, int nwg, double2* tl) {
-    const int Dl = dims[p], mid = dims[p + 1], Dr = dims[p + 2];
-    const int sA = cap[p + 1], sC = cap[p + 2];
-    const std::size_t oA = off[p], oC = off[p + 1];
-    const int slot = threadIdx.x >> 8, nslot = blockDim.x >> 8;
-    const int t16 = threadIdx.x & 255, tx = t16 & 15, ty = t16 >> 4;
-    double2* As = tl + slot * (2 * 16 * 2 * 17);  // [16][2][17]
-    double2* Cs = As + 16 * 2 * 17;
-    const int nta = (Dl + 15) / 16, ntb = (Dr + 15) / 16, total = nta * ntb;
-    const bool tall = Dr <= Dl;
-    for (int base = rank * nslot; base < total; base += nwg * nslot) {
-        const int tile = base + slot;
-        const bool active = tile < total;
-        const int ta = active ? tile / ntb : 0, tb = active ? tile % ntb : 0;
-        const int a = ta * 16 + ty, b = tb * 16 + tx;
-        double2 th[2][2] = {{make_double2(0, 0), make_double2(0, 0)}, {make_double2(0, 0), make_double2(0, 0)}};
-        for (int c0 = 0; c0 < mid; c0 += 16) {
-            for (int s = 0; s < 2; ++s) {
-                const int cc = c0 + tx, cr = c0 + ty;
-                As[(ty * 2 + s) * 17 + tx] =
-                    (active && a < Dl && cc < mid) ? ld_site(sites, oA, sA, a, s, cc) : make_double2(0, 0);
-                Cs[(ty * 2 + s) * 17 + tx] =
-                    (active && cr < mid && b < Dr) ? ld_site(sites, oC, sC, cr, s, b) : make_double2(0, 0);
-            }
-            __syncthreads();
-            for (int cc = 0; cc < 16; ++cc)
-                for (int s1 = 0; s1 < 2; ++s1) {
-                    const double2 x = As[(ty * 2 + s1) * 17 + cc];
-                    th[s1][0] = cadd(th[s1][0], cmul(x, Cs[(cc * 2 + 0) * 17 + tx]));
-                    th[s1][1] = cadd(th[s1][1], cmul(x, Cs[(cc * 2 + 1) * 17 + tx]));
-                }
-            __syncthreads();
-        }
-        if (active && a < Dl && b < Dr)
-            for (int r1 = 0; r1 < 2; ++r1)
-                for (int r2 = 0; r2 < 2; ++r2) {
-                    double2 acc = make_double2(0, 0);
-                    for (int s1 = 0; s1 < 2; ++s1)
-                        for (int s2 = 0; s2 < 2; ++s2)
-                            acc = cadd(acc, cmul(g.g[(r1 * 2 + r2) * 4 + s1 * 2 + s2], th[s1][s2]));
-                    const int row = a * 2 + r1, col = r2 * Dr + b;
-                    const std::size_t at = tall ? (std::size_t)col * ld + row : (std::size_t)row * ld + col;
-                    const double2 v = tall ? acc : cconj(acc);
-                    M[at] = v;
-                    W[at] = v;
-                }
-    }
+                            int ld, int rank, int nwg) {
+    const int wpg = (int)(blockDim.x >> 6);
+    theta_mfma(sites, off, cap, dims, p, g, M, W, ld, (long long)rank * wpg + (threadIdx.x >> 6),
+               (long long)nwg * wpg);
 }
 
 // ---------------------------------------------------------------------------
@@ -1725,7 +1753,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
             if (a.snap_begin[st + 1] > a.snap_begin[st]) sync();
             if (active) {
                 BlockSync bs;
-                theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, 0, 1, lds);
+                theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, 0, 1);
                 bs();
                 const int R = 2 * I.dims[u.site], C = 2 * I.dims[u.site + 2];
                 const int m = C <= R ? R : C, c = C <= R ? C : R;
@@ -1762,7 +1790,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 // theta, SVD and reconstruction of this group's update on the
                 // group's own barrier: own tournament length and convergence
                 GroupSync<XL> gs{a.group_bar + 2 * grp, nwg, a.err, a.spin_limit};
-                theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
+                theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg);
                 mark(0);
                 SvdMode mode;
                 mode.prof = blockIdx.x == 0;
@@ -1793,7 +1821,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 mark(6);
             }
         } else {
-            if (active) theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg, lds);
+            if (active) theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg);
             mark(0);
             // 2-3. lockstep SVD; svd_blocks opens with a grid barrier
             SvdMode mode;
@@ -2054,9 +2082,9 @@ void Tebd::randomize(uint64_t seed, double decay) {
 void Tebd::two_site(int p, const Gate4& g, const double* wts) {
     const int ld = ld_;
     const int Dlc = cap_[p], Drc = cap_[p + 2];
-    dim3 tgrid((Drc + 15) / 16, (Dlc + 15) / 16);
-    hipLaunchKernelGGL(k_theta, tgrid, dim3(16, 16), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p, p, g,
-                       mat_.p, work_.p, ld);
+    const long long ttiles = theta_tiles_for(Dlc, Drc);  // one wave per tile at full capacity
+    hipLaunchKernelGGL(k_theta, dim3((unsigned)((ttiles + 3) / 4)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
+                       caps_dev_.p, dims_.p, p, g, mat_.p, work_.p, ld);
     TQ_LAUNCH_CHECK();
     // worst-case matrix at this bond decides the kernel (both dims <= 2*cap)
     const std::size_t bytes = (std::size_t)(2 * Dlc) * (2 * Drc) * sizeof(double2);
@@ -2221,10 +2249,9 @@ static void exec_shape(int maxcap, int req, bool solo, bool group_sync, int work
     // workers take several pairs each (fewer workgroups in every barrier)
     per_group = solo ? 1 : (workers > 0 ? std::min(workers, nbp / 2) : nbp / 2);
     const std::size_t svd_bytes = (std::size_t)2 * kBlk * (2 * maxcap) * sizeof(double2);
-    const std::size_t theta_bytes = (std::size_t)(kDistThreads / 256) * 2 * 16 * 2 * 17 * sizeof(double2);
     // solo: room for a kSmall x kSmall theta of svd_small
     const std::size_t small_bytes = solo ? (std::size_t)kSmall * kSmall * sizeof(double2) : 0;
-    shm = std::max(std::max(svd_bytes, theta_bytes), small_bytes);
+    shm = std::max(svd_bytes, small_bytes);
     int per_cu = 0, cus = 0, dev = 0;
     TQ_HIP(hipGetDevice(&dev));
     TQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -2835,9 +2862,9 @@ std::vector<double> Tebd::theta(int p, int& rows, int& cols) {
     const int Dl = d[p], Dr = d[p + 2];
     rows = 2 * Dl;
     cols = 2 * Dr;
-    dim3 tgrid((cap_[p + 2] + 15) / 16, (cap_[p] + 15) / 16);
-    hipLaunchKernelGGL(k_theta, tgrid, dim3(16, 16), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p, p, eye,
-                       mat_.p, work_.p, ld_);
+    const long long ttiles = theta_tiles_for(cap_[p], cap_[p + 2]);
+    hipLaunchKernelGGL(k_theta, dim3((unsigned)((ttiles + 3) / 4)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
+                       caps_dev_.p, dims_.p, p, eye, mat_.p, work_.p, ld_);
     TQ_LAUNCH_CHECK();
     std::vector<double2> m((std::size_t)ld_ * ld_);
     TQ_HIP(hipMemcpyAsync(m.data(), mat_.p, m.size() * sizeof(double2), hipMemcpyDeviceToHost, stream_));

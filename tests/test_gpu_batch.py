@@ -27,11 +27,23 @@ def _mixed():
     ]
 
 
+def _rank_at_rounding_level(model, sched, dt, chi):
+    """Whether the oracle meets kept-rank decisions set by singular values at the
+    rounding level of theta (between 1e-15 and 1e-13, next to the 1e-14 cut of
+    mps.cpp:82) on this run: there two accurate SVDs may keep different numbers
+    of directions of weight ~1e-28, and only the states are comparable."""
+    import pyoracle
+    return int(np.sum(pyoracle.tebd(model, sched, dt, chi, svd_mode=1)["ambiguous"])) > 0
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_batch_matches_one_at_a_time(mode):
     """mode 0: automatic layout (grouped here: bonds up to 64), 1: grouped, 2: solo
-    (thetas up to 64 x 64 in LDS, larger ones through the block SVD)."""
+    (thetas up to 64 x 64 in LDS, larger ones through the block SVD). Bond
+    dimensions are compared exactly unless the instance has rank decisions at the
+    rounding level (the states are compared in every case)."""
     cases = _mixed()
+    amb = [_rank_at_rounding_level(*c) for c in cases]
     solo = []
     for model, sched, dt, chi in cases:
         s = qa.TebdSession(model, chi, dt)
@@ -47,9 +59,11 @@ def test_batch_matches_one_at_a_time(mode):
         spins, sc = s.readout()
         psi = s.dense()
         inf = 1 - _fid(solo[k][0], psi)
-        print(f"instance {k}: infidelity={inf:.2e} trunc={tr:.2e} bonds={bonds.max()}", flush=True)
+        print(f"instance {k}: infidelity={inf:.2e} trunc={tr:.2e} bonds={bonds.max()} "
+              f"(one at a time {solo[k][2].max()}; rounding-level ranks: {amb[k]})", flush=True)
         assert inf <= 1e-12
-        assert np.array_equal(bonds, solo[k][2])
+        if not amb[k]:
+            assert np.array_equal(bonds, solo[k][2])
         assert tr <= 1e-20 and solo[k][1] <= 1e-20
         assert np.array_equal(spins, solo[k][3])
         assert abs(sc[1] - solo[k][4][1]) <= 1e-10
@@ -87,7 +101,8 @@ def test_batch_config4_sample_runs():
     (~1e-14 of its norm), and the two SVD variants (in-LDS pair Jacobi for
     small thetas, LQ-preconditioned block Jacobi) round them to different sides
     of the S < 1e-14 keep cut (mps.cpp:77-85); what either drops weighs below
-    1e-20. (On the mixed exact cases above the ranks are identical.)"""
+    1e-20. (On the mixed exact cases above the ranks are identical wherever the
+    oracle sees no rank decision at the rounding level.)"""
     insts = I.batch_instances(8)
     scheds = [I.Schedule(x.sched.a[:3], x.sched.b[:3]) for x in insts]
     res = []

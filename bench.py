@@ -327,6 +327,9 @@ def batch_bench(args):
         s.sync()
     t_batch = time.perf_counter() - t0
     ex = ss[0].exec_stats()
+    # per-instance completion inside the batched launch (device clock: launch start to
+    # the end of the super-step holding the instance's last update)
+    batched_ms = np.array([s.last_run_ms() for s in ss])
     for s in ss:
         s.close()
     progress(f"batched: {t_batch:.2f}s")
@@ -344,7 +347,9 @@ def batch_bench(args):
                    "speedup": t_solo / t_batch,
                    "one_at_a_time_instance_s_p50": float(np.percentile(st, 50)),
                    "one_at_a_time_instance_s_p99": float(np.percentile(st, 99)),
-                   "batched_instance_s": t_batch,
+                   "batched_instance_s_p50": float(np.percentile(batched_ms, 50)) * 1e-3,
+                   "batched_instance_s_p99": float(np.percentile(batched_ms, 99)) * 1e-3,
+                   "batched_launch_s": t_batch,
                    "exec_mode": mode, "supersteps": ex["supersteps"],
                    "updates_per_superstep": ex["updates"] / max(ex["supersteps"], 1)},
     }
@@ -590,10 +595,22 @@ def tebd_timed(args, model, sched, k0, sess):
         ex1 = sess.exec_stats()
         ss = ex1["supersteps"] - ex0["supersteps"]
         kname = "k_tebd_lockstep"
-        kernel = ("k_tebd_lockstep (super-steps of theta, block-Jacobi SVDs on per-group barriers with "
-                  "point-to-point tournament rounds, reconstruction)")
+        kernel = ("k_tebd_lockstep (super-steps of theta and reconstruction on FP64 MFMA, block-Jacobi SVDs "
+                  "(FP64 VALU) on per-group barriers with point-to-point tournament rounds)")
         pr1 = sess.exec_profile()
-        timing = {"kernel_ms": kernel_ms, "kernel_share_of_wall": kernel_ms * 1e-3 / wall, "supersteps": ss,
+        # the theta and reconstruction phases alone (workgroup 0's timers cover its
+        # group's updates): F of those updates / phase time, against the FP64 MFMA
+        # peak of the group's CUs (one workgroup per CU)
+        nb = (2 * args.chi + 7) // 8
+        per_group = args.exec_workers if args.exec_workers > 0 else max(2, nb + (nb & 1)) // 2
+        g_updates = pr1["active_supersteps"] - pr0["active_supersteps"]
+        g_ms = (pr1["theta_ms"] - pr0["theta_ms"]) + (pr1["recon_ms"] - pr0["recon_ms"])
+        g_tflops = g_updates * (f_step / max(ups_host, 1)) / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+        g_peak = FP64_PEAK_TFLOPS * per_group / 256.0
+        timing = {"gemm_phases": {"tflops_per_group": g_tflops, "group_peak_tflops": g_peak,
+                                  "frac_of_group_peak": g_tflops / g_peak,
+                                  "note": "theta + reconstruction phases of workgroup 0's group on v_mfma_f64_16x16x4"},
+                  "kernel_ms": kernel_ms, "kernel_share_of_wall": kernel_ms * 1e-3 / wall, "supersteps": ss,
                   "updates_per_superstep": (ex1["updates"] - ex0["updates"]) / max(ss, 1),
                   "group_barriers": os.environ.get("TESSERA_EXEC_LOCKSTEP", "0") != "1",
                   "point_to_point_rounds": (os.environ.get("TESSERA_EXEC_LOCKSTEP", "0") != "1"

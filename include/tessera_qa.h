@@ -220,6 +220,12 @@ int tessera_tebd_batch_run(tessera_tebd* const* sessions, size_t count, const do
  * concurrent updates (-1: as many as fit). Results agree to rounding. */
 int tessera_tebd_batch_run_ex(tessera_tebd* const* sessions, size_t count, const double* const* a,
                               const double* const* b, const size_t* steps, int groups, int mode);
+/* Per-instance wall time inside the last executor launch (single or batched)
+ * that advanced this session: device-clock milliseconds from the launch start
+ * to the end of the super-step holding the session's last update (0 if that
+ * launch gave it no update, or ran in the dataflow order). No upstream
+ * counterpart: the batched scheduler's per-instance latency. */
+int tessera_tebd_last_run_ms(tessera_tebd* t, double* out_ms);
 /* Lockstep executor totals since creation: super-steps run and two-site
  * updates placed in them (updates / (super-steps * groups) = slot fill). */
 int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* out_updates);

@@ -547,6 +547,17 @@ int tessera_tebd_exec_profile(tessera_tebd* t, double* out, size_t count) {
     }
 }
 
+int tessera_tebd_last_run_ms(tessera_tebd* t, double* out_ms) {
+    try {
+        if (!t || !out_ms) throw Error(TESSERA_ERR_ARG, "null argument");
+        *out_ms = t->eng->last_run_ms();
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
 int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* out_updates) {
     try {
         if (!t) throw Error(TESSERA_ERR_ARG, "null session");

@@ -1604,6 +1604,7 @@ struct ExecArgs {
                         // waits for the done flags of its predecessors instead
     int* done;          // [updates] done flags of the dataflow order
     unsigned long long* prof;  // [kProfSlots] phase profile of workgroup 0 (accumulated)
+    unsigned long long* ss_clock;  // [nsteps + 1] or null: wall clock at the end of each super-step, then at the start
 };
 
 // Barrier of the nwg co-resident workgroups of one executor group (the
@@ -1715,6 +1716,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
     if (prof)
         for (int q = 0; q < kProfSlots; ++q) sh.tm[q] = 0;
     const unsigned long long t_start = prof ? wall_clock64() : 0;
+    if (a.ss_clock && prof) a.ss_clock[a.nsteps] = t_start;
     unsigned long long tp = t_start;
     auto mark = [&](int slot) {
         if (prof) {
@@ -1843,6 +1845,9 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
             sync();
             mark(6);
         }
+        // every layout has just passed the super-step's closing grid barrier (the
+        // dataflow order has none and records no clock)
+        if (a.ss_clock && blockIdx.x == 0 && tid == 0) a.ss_clock[st] = wall_clock64();
     }
     if (prof) {
         sh.tm[10] += wall_clock64() - t_start;
@@ -2330,6 +2335,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     std::vector<SnapAt> snaps;
     int nsteps = 0;
     std::vector<std::vector<M2>> pendings(ts.size());
+    std::vector<int> last_ss(ts.size(), -1);  // last super-step holding an update of each instance
     // open[t]: a super-step >= t with a free slot (path-compressed), so that
     // placing the updates of many instances stays linear in their number
     std::vector<int> open;
@@ -2368,6 +2374,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
         };
         std::vector<M2>& pending = pendings[in];
         pending.assign(n, eye);
+        last_ss[in] = -1;
         std::vector<int> free_at(n, 0);   // first super-step at which the site may be updated
         std::vector<int> last_at(n, -1);  // super-step of the site's latest update
         // (super-step << 16 | slot) of the site's latest update, -1: none; converted
@@ -2392,6 +2399,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
             ups[t].push_back(u);
             if ((int)ups[t].size() >= G) open[t] = t + 1;  // full: later searches skip it
             rounds[t] = std::max(rounds[t], nbp_of(p) - 1);
+            last_ss[in] = std::max(last_ss[in], t);
             free_at[p] = free_at[p + 1] = t + 1;
             nsteps = std::max(nsteps, t + 1);
             return t;
@@ -2575,6 +2583,11 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     }
     x.group_bar = t0.ex_bar_.p + 1;
     x.prof = t0.ex_prof_.p;
+    x.ss_clock = nullptr;
+    if (!x.dataflow) {
+        if (t0.ex_clock_.n < (std::size_t)total_steps + 1) t0.ex_clock_.alloc((std::size_t)total_steps + 1);
+        x.ss_clock = t0.ex_clock_.p;
+    }
     void* args[] = {(void*)&x};
     hipStream_t st = t0.stream_;
     // barrier counters and the error word start at zero (ordinary stream work:
@@ -2622,6 +2635,19 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
             t0.entropy_of(t0.ex_snap_.p + (std::size_t)k * t0.off_[t0.n_],
                           t0.ex_sdims_.p + (std::size_t)k * (t0.n_ + 1), rec0 + k);
     TQ_HIP(hipStreamSynchronize(st));  // host staging buffers go out of scope
+    // per instance: device time from the launch start to the end of the super-step
+    // holding its last update
+    for (Tebd* t : ts) t->last_run_ms_ = 0.0;
+    if (x.ss_clock && total_steps > 0) {
+        std::vector<unsigned long long> clk((std::size_t)total_steps + 1);
+        TQ_HIP(hipMemcpy(clk.data(), x.ss_clock, clk.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        int dev = 0, khz = 0;
+        TQ_HIP(hipGetDevice(&dev));
+        TQ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        for (std::size_t in = 0; in < ts.size(); ++in)
+            if (last_ss[in] >= 0 && khz > 0)
+                ts[in]->last_run_ms_ = (double)(clk[last_ss[in]] - clk[total_steps]) / khz;
+    }
 }
 
 void Tebd::env_right(const double2* rin, int j, double2* rout, const double2* sites, const int* dims) {

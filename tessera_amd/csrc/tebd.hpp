@@ -89,6 +89,10 @@ public:
     std::vector<double> dense();                 // n <= 24, interleaved re/im
     std::vector<double> theta(int p, int& rows, int& cols);  // identity-gate theta at (p, p+1)
     std::size_t updates_per_step() const { return plan_.size() * (std::size_t)ns_; }
+    // Device time (ms) from the start of the last executor launch that advanced
+    // this session to the end of the super-step holding its last update (0 when
+    // that launch gave it no update or ran in the dataflow order).
+    double last_run_ms() const { return last_run_ms_; }
     int n() const { return n_; }
     hipStream_t stream() const { return stream_; }
     // Capacity of bond b (0..n): min(chi, 2^b, 2^(n-b)).
@@ -156,6 +160,8 @@ private:
     int exec_req_ = 0;
     int exec_workers_ = 0;
     long long exec_supersteps_ = 0, exec_updates_ = 0;
+    double last_run_ms_ = 0.0;             // see last_run_ms()
+    DevBuf<unsigned long long> ex_clock_;  // super-step end clocks of the last launch
     DevBuf<unsigned char> ex_tasks_;   // schedule: updates, snapshots, rounds, snapshot offsets
     DevBuf<double2> ex_mat_, ex_work_, ex_snap_, ex_gates_, ex_aux_, ex_taus_;
     DevBuf<double> ex_sv_;

@@ -95,6 +95,8 @@ def lib() -> ctypes.CDLL:
         L.tessera_tebd_batch_run.argtypes = [vp, _usz, vp, vp, vp, ctypes.c_int]
         L.tessera_tebd_batch_run_ex.restype = ctypes.c_int
         L.tessera_tebd_batch_run_ex.argtypes = [vp, _usz, vp, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.tessera_tebd_last_run_ms.restype = ctypes.c_int
+        L.tessera_tebd_last_run_ms.argtypes = [vp, vp]
         L.tessera_tebd_exec_stats.restype = ctypes.c_int
         L.tessera_tebd_exec_stats.argtypes = [vp, vp, vp]
         L.tessera_tebd_exec_profile.restype = ctypes.c_int
@@ -436,6 +438,14 @@ class TebdSession:
         _check(lib().tessera_tebd_theta(self._h, p, _p(out), r.value * c.value, ctypes.addressof(r),
                                         ctypes.addressof(c)))
         return (out[0::2] + 1j * out[1::2]).reshape(r.value, c.value)
+
+    def last_run_ms(self) -> float:
+        """Device ms from the start of the last executor launch that advanced this
+        session to the end of the super-step holding its last update (the
+        per-instance wall time inside a batched run)."""
+        out = ctypes.c_double()
+        _check(lib().tessera_tebd_last_run_ms(self._h, ctypes.addressof(out)))
+        return out.value
 
     def exec_stats(self) -> dict:
         """Lockstep executor totals: super-steps run and updates placed in them."""

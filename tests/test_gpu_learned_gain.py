@@ -1,0 +1,86 @@
+"""What the trained guidance network gains over the method without it.
+
+The network is trained on targets the GPU TEBD engine itself produces (final
+<Z_i> and entanglement-peak time of plain batched anneals of seeded training
+instances), by the library's BF16 MFMA forward/backward kernels and Adam. On
+fixed-seed held-out instances -- including one at n = 128, chi = 128 -- three
+anneals of the same shortened length (half the plain length; 100 plain steps,
+40 for the n = 128 instance) are compared with
+the best plain energy of each instance (the better of the full-length and the
+half-length plain anneal; plain TEBD is the judge):
+
+* plain TEBD at half length,
+* guided by the trained network (warm-start bias b = z_pred, the same length),
+* guided by an untrained network of the same architecture (random weights).
+
+Stated margins: the trained guidance must reach the best plain energy within
+1% on at least 25 percentage points more held-out instances than plain TEBD
+and than the untrained network, and its mean relative energy gap must be below
+half of theirs. The reference's analogue is the GNN-vs-random test of
+python/test_gnn.py:17-47.
+"""
+import numpy as np
+import pytest
+
+from tessera_amd import learned as Lrn
+
+pytestmark = pytest.mark.gpu
+
+
+def _gap(e, best):
+    return (np.asarray(e) - best) / np.abs(best)
+
+
+def test_trained_guidance_beats_plain_and_untrained_at_half_length():
+    steps, dt = 100, 0.05
+    train = Lrn.guidance_instances(96, 5_000_000, 12, 24, chi=16, steps=steps, dt=dt)
+    tg = Lrn.tebd_targets(train, chunks=5)
+    print(f"targets for {len(train)} training instances: {tg.seconds:.1f}s", flush=True)
+    batch = Lrn.pack([x.model for x in train], tg.z, tg.s_peak)
+    with Lrn.GuidanceNet(seed=5) as net, Lrn.GuidanceNet(seed=11) as untrained:
+        net.set_data(batch)
+        hist = net.fit(80, batch_graphs=32, lr=2e-3, seed=0)
+        print(f"training loss {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
+        assert hist[-1] < 0.7 * hist[0]
+        small = Lrn.guidance_instances(24, 6_000_000, 24, 48, chi=32, steps=steps, dt=dt)
+        # one held-out instance at the batched workload's largest scale (n = 128, chi = 128),
+        # on a 40-step schedule so that its four anneals fit the test's time; annealed in
+        # batches of its own (the chi = 32 ones take the one-workgroup-per-update layout)
+        big = Lrn.guidance_instances(1, 6_100_000, 128, 128, chi=128, steps=40, dt=dt)
+        evals = small + big
+
+        def anneal(steps_, bias=None, what=""):
+            e, w = [], 0.0
+            for part, lo in ((small, 0), (big, len(small))):
+                hi = lo + len(part)
+                ee, _, ww = Lrn.anneal_batch(part, steps_[lo:hi], None if bias is None else bias[lo:hi])
+                e.append(ee)
+                w += ww
+                print(f"{what}: {hi}/{len(evals)} instances, {w:.1f}s", flush=True)
+            return np.concatenate(e), w
+
+        models = [x.model for x in evals]
+        base = np.array([x.sched.steps for x in evals])
+        half = np.maximum(2, base // 2)
+        e_full, w_full = anneal(base, what="plain full length")
+        e_half, w_half = anneal(half, what="plain half length")
+        best = np.minimum(e_full, e_half)
+        s_mean = float(tg.s_peak.mean())
+        g_tr = Lrn.guidance(net, models, base, 0.5, 1.0, s_mean, stretch_range=(1.0, 1.0))
+        g_un = Lrn.guidance(untrained, models, base, 0.5, 1.0, s_mean, stretch_range=(1.0, 1.0))
+        assert np.array_equal(g_tr.steps, half) and np.array_equal(g_un.steps, half)
+        e_tr, w_tr = anneal(g_tr.steps, g_tr.bias, what="guided (trained) half length")
+        e_un, _ = anneal(g_un.steps, g_un.bias, what="guided (untrained) half length")
+    ok = {k: Lrn.within(e, best).mean() for k, e in (("plain", e_half), ("trained", e_tr), ("untrained", e_un))}
+    gap = {k: _gap(e, best).mean() for k, e in (("plain", e_half), ("trained", e_tr), ("untrained", e_un))}
+    print(f"held-out {len(evals)} (n 24..48 chi 32, and n=128 chi=128) at half length: within 1% "
+          + ", ".join(f"{k} {v:.2f}" for k, v in ok.items()) + "; mean gap "
+          + ", ".join(f"{k} {v:.4f}" for k, v in gap.items())
+          + f"; full-length plain {w_full:.1f}s, half {w_half:.1f}s, guided {w_tr:.1f}s", flush=True)
+    print(f"n=128 chi=128: best {best[-1]:.6f}, plain half {e_half[-1]:.6f}, trained {e_tr[-1]:.6f}, "
+          f"untrained {e_un[-1]:.6f}", flush=True)
+    assert np.all(np.isfinite(e_tr)) and np.all(np.isfinite(e_un))
+    assert ok["trained"] >= ok["plain"] + 0.25
+    assert ok["trained"] >= ok["untrained"] + 0.25
+    assert gap["trained"] < 0.5 * gap["plain"]
+    assert gap["trained"] < 0.5 * gap["untrained"]

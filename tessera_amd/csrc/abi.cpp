@@ -91,13 +91,9 @@ int tessera_qa_anneal_ex(const double* h, size_t n, const size_t* ei, const size
         if (chi == 0) throw Error(TESSERA_ERR_ARG, "chi == 0");
         if (steps > (size_t)INT32_MAX) throw Error(TESSERA_ERR_ARG, "too many steps");
         Tebd t(make_model(h, n, ei, ej, ew, m, bias), (int)chi, dt, zz_substeps);
-        // device-side traces sized once: the step loop only enqueues work
-        t.ensure_trace((int)steps);
-        if (out_max_bond) t.ensure_max_bond((int)steps);
-        for (size_t k = 0; k < steps; ++k) {
-            t.step(schedule_a[k], schedule_b[k], (int)k);
-            if (out_max_bond) t.record_max_bond((int)k);
-        }
+        // the per-update path (one captured Trotter step replayed per step where every
+        // update's SVD takes a single-workgroup kernel)
+        t.run(schedule_a, schedule_b, (int)steps, 0, out_max_bond ? 0 : -1);
         t.sync();
         if (out_max_bond && steps) {
             const std::vector<int> mb = t.max_bonds((int)steps);

@@ -86,8 +86,13 @@ __device__ __forceinline__ unsigned xcc_id() {
 // exp(-i dt/2 g), g = [[-hz, -A], [-A, hz]], hz = B h_i (+ A bias_i), closed form.
 __global__ void __launch_bounds__(256) k_single_layer(double2* sites, const std::size_t* off, const int* cap,
                                                       const int* dims, const double* h, const double* bias,
-                                                      double A, double B, double dt) {
+                                                      double A, double B, double dt, StepParams sp) {
     const int i = blockIdx.x;
+    if (sp.ab) {  // replayed step: this step's schedule values
+        const int k = sp.par[0];
+        A = sp.ab[2 * k];
+        B = sp.ab[2 * k + 1];
+    }
     const int Dl = dims[i], Dr = dims[i + 1], stride = cap[i + 1];
     const double hz = B * h[i] + (bias ? A * bias[i] : 0.0);
     const double r = sqrt(A * A + hz * hz);
@@ -123,7 +128,8 @@ __global__ void __launch_bounds__(256) k_single_layer(double2* sites, const std:
 // `wave`, `wave + nwaves`, ...
 __device__ void theta_mfma(const double2* __restrict__ sites, const std::size_t* off, const int* cap,
                            const int* dims, int p, const Gate4& g, double2* __restrict__ M,
-                           double2* __restrict__ W, int ld, long long wave, long long nwaves) {
+                           double2* __restrict__ W, int ld, long long wave, long long nwaves,
+                           const double2* zzc = nullptr) {
     const int Dl = dims[p], mid = dims[p + 1], Dr = dims[p + 2];
     const int sA = cap[p + 1], sC = cap[p + 2];
     const double2* A = sites + off[p];
@@ -135,9 +141,20 @@ __device__ void theta_mfma(const double2* __restrict__ sites, const std::size_t*
     // the gate in LDS: the epilogue reads its coefficients where it needs them
     // (kept in registers across the tile loop they would crowd out the products)
     __shared__ double2 gl[16];
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
+        if (zzc) {  // exp(-i theta Z(x)Z) from the step's phasor (cos theta, sin theta)
+            const double2 cs = *zzc;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) gl[q] = g.g[q];
+            for (int q = 0; q < 16; ++q) gl[q] = make_double2(0, 0);
+            gl[0] = make_double2(cs.x, -cs.y);
+            gl[5] = make_double2(cs.x, cs.y);
+            gl[10] = make_double2(cs.x, cs.y);
+            gl[15] = make_double2(cs.x, -cs.y);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) gl[q] = g.g[q];
+        }
+    }
     __syncthreads();
     for (long long t = wave; t < (long long)nta * ntb; t += nwaves) {
         const int a0 = (int)(t / ntb) * 8, b0 = (int)(t % ntb) * 16;
@@ -196,11 +213,17 @@ inline long long theta_tiles_for(int Dl, int Dr) { return (long long)((Dl + 7) /
 // capacities of the bonds (tiles outside the current dimensions do nothing).
 __global__ void __launch_bounds__(256) k_theta(const double2* __restrict__ sites, const std::size_t* off,
                                                const int* cap, const int* dims, int p, Gate4 g,
-                                               double2* __restrict__ M, double2* __restrict__ W, int ld) {
+                                               double2* __restrict__ M, double2* __restrict__ W, int ld,
+                                               StepParams sp, int edge) {
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-    theta_mfma(sites, off, cap, dims, p, g, M, W, ld, wave, nwaves);
+    // replayed step with a ZZ gate: the gate of this step from the phasor table
+    const double2* zzc = (sp.zz && edge >= 0) ? sp.zz + (std::size_t)sp.par[0] * sp.m + edge : nullptr;
+    theta_mfma(sites, off, cap, dims, p, g, M, W, ld, wave, nwaves, zzc);
 }
+
+// Advances a replayed step's index (the last node of a captured Trotter step).
+__global__ void k_step_advance(int* par) { par[0] += 1; }
 
 // ---- one-sided Jacobi SVD, one workgroup -----------------------------------
 __device__ inline void rr_pair(int r, int k, int cp, int& i, int& j) {
@@ -1420,8 +1443,9 @@ __global__ void k_set_unit(double2* e) { e[0] = make_double2(1.0, 0.0); }
 // Qubit-0 reduced density matrix from the right environment of site 1 and the
 // closed-form 2x2 entropy (mps.cpp:263-290).
 __global__ void k_rho0(const double2* sites, const std::size_t* off, const int* cap, const int* dims,
-                       const double2* R1, int ldr, double* out, int slot) {
+                       const double2* R1, int ldr, double* out, int slot, const int* par = nullptr) {
     if (threadIdx.x != 0) return;
+    if (par) slot += par[1] + par[0];  // replayed step: its recorded slot
     const int Dr = dims[1], st = cap[1];
     double2 rho[2][2];
     for (int s = 0; s < 2; ++s)
@@ -1963,6 +1987,8 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     // XCD-locally when the groups sit one per XCD. Its first run at n = 128,
     // chi = 256 left the GPU faulted (smaller runs were exact); the cause is not
     // established, so the default keeps device-scope barriers.
+    const char* gr = std::getenv("TESSERA_STEP_GRAPH");  // knob: 0 launches every update of a step directly
+    graph_on_ = !(gr && gr[0] == '0');
     const char* xl = std::getenv("TESSERA_EXEC_XCD_LOCAL");
     ex_xcd_local_ = xl && xl[0] == '1';
     const char* xp = std::getenv("TESSERA_SVD_XCD_PACK");  // 1: keep the SVD's workgroups on one XCD
@@ -1972,6 +1998,7 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
 }
 
 Tebd::~Tebd() {
+    drop_graph();
     for (auto& e : events_) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
@@ -1992,6 +2019,17 @@ void Tebd::build_plan() {
         for (int k = i + 1; k < j; ++k) plan_.push_back({k, -1});
         spans_.push_back({plan_.size() - (std::size_t)(2 * (j - i) - 1), plan_.size()});
     }
+    // a step can be captured into a graph when no update needs the cooperative
+    // (multi-workgroup) SVD launch; the choice below mirrors two_site's
+    plan_graphable_ = true;
+    for (const Update& u : plan_) {
+        const int Dlc = cap_[u.site], Drc = cap_[u.site + 2];
+        const std::size_t bytes = (std::size_t)(2 * Dlc) * (2 * Drc) * sizeof(double2);
+        const int ccap = std::min(2 * Dlc, 2 * Drc), mcap = std::max(2 * Dlc, 2 * Drc);
+        const bool small = small_svd_ && !force_dist_ && ccap <= kSmall && mcap <= kSmallTall;
+        const bool wg = bytes <= 144 * 1024 && !force_dist_;
+        plan_graphable_ = plan_graphable_ && (small || wg);
+    }
 }
 
 void Tebd::ensure_trace(int count) {
@@ -2005,11 +2043,12 @@ void Tebd::ensure_trace(int count) {
 }
 
 // Largest interior bond dims[1..n-1] (at least 1) into *out: one wavefront.
-__global__ void k_max_bond(const int* __restrict__ dims, int n, int* __restrict__ out) {
+// In a replayed step (par non-null) the slot is par[2] + par[0].
+__global__ void k_max_bond(const int* __restrict__ dims, int n, int* __restrict__ out, const int* par) {
     int m = 1;
     for (int i = 1 + (int)threadIdx.x; i < n; i += 64) m = max(m, dims[i]);
     for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
-    if (threadIdx.x == 0) *out = m;
+    if (threadIdx.x == 0) out[par ? par[2] + par[0] : 0] = m;
 }
 
 void Tebd::ensure_max_bond(int count) {
@@ -2022,7 +2061,8 @@ void Tebd::ensure_max_bond(int count) {
 
 void Tebd::record_max_bond(int slot) {
     if (slot < 0 || slot >= mbond_cap_) throw Error(-1, "max-bond slot out of range");
-    hipLaunchKernelGGL(k_max_bond, dim3(1), dim3(64), 0, stream_, dims_.p, n_, mbond_.p + slot);
+    hipLaunchKernelGGL(k_max_bond, dim3(1), dim3(64), 0, stream_, dims_.p, n_, capturing_ ? mbond_.p : mbond_.p + slot,
+                       capturing_ ? sp_.par : nullptr);
     TQ_LAUNCH_CHECK();
 }
 
@@ -2084,12 +2124,12 @@ void Tebd::randomize(uint64_t seed, double decay) {
     TQ_HIP(hipStreamSynchronize(stream_));
 }
 
-void Tebd::two_site(int p, const Gate4& g, const double* wts) {
+void Tebd::two_site(int p, const Gate4& g, const double* wts, int edge) {
     const int ld = ld_;
     const int Dlc = cap_[p], Drc = cap_[p + 2];
     const long long ttiles = theta_tiles_for(Dlc, Drc);  // one wave per tile at full capacity
     hipLaunchKernelGGL(k_theta, dim3((unsigned)((ttiles + 3) / 4)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
-                       caps_dev_.p, dims_.p, p, g, mat_.p, work_.p, ld);
+                       caps_dev_.p, dims_.p, p, g, mat_.p, work_.p, ld, sp_, edge);
     TQ_LAUNCH_CHECK();
     // worst-case matrix at this bond decides the kernel (both dims <= 2*cap)
     const std::size_t bytes = (std::size_t)(2 * Dlc) * (2 * Drc) * sizeof(double2);
@@ -2158,14 +2198,14 @@ static Gate4 make_zz_gate(double theta) {
 void Tebd::step(double A, double B, int rec) {
     const double* bias = bias_.p;
     hipLaunchKernelGGL(k_single_layer, dim3(n_), dim3(256), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p,
-                       h_.p, bias, A, B, dt_);
+                       h_.p, bias, A, B, dt_, sp_);
     TQ_LAUNCH_CHECK();
     static const Gate4 sw = make_swap_gate();
     const double sub = dt_ / ns_;
     auto run_edge = [&](const std::pair<std::size_t, std::size_t>& span) {
         for (std::size_t q = span.first; q < span.second; ++q) {
             const Update& u = plan_[q];
-            two_site(u.site, u.edge < 0 ? sw : make_zz_gate(-B * model_.ew[u.edge] * sub));
+            two_site(u.site, u.edge < 0 ? sw : make_zz_gate(-B * model_.ew[u.edge] * sub), nullptr, u.edge);
         }
     };
     for (int s = 0; s < ns_; ++s) {
@@ -2177,15 +2217,16 @@ void Tebd::step(double A, double B, int rec) {
             for (std::size_t k = spans_.size(); k-- > 0;) run_edge(spans_[k]);
     }
     hipLaunchKernelGGL(k_single_layer, dim3(n_), dim3(256), 0, stream_, sites_.p, off_dev_.p, caps_dev_.p, dims_.p,
-                       h_.p, bias, A, B, dt_);
+                       h_.p, bias, A, B, dt_, sp_);
     TQ_LAUNCH_CHECK();
     if (rec >= 0) entropy_sweep(rec);
 }
 
-void Tebd::run(const double* a, const double* b, int count, int rec0) {
+void Tebd::run(const double* a, const double* b, int count, int rec0, int bond0) {
     if (count <= 0) return;
     if (rec0 >= 0) ensure_trace(rec0 + count);
-    if (exec_req_ != 0 && n_ >= 2) {
+    if (bond0 >= 0) ensure_max_bond(bond0 + count);
+    if (exec_req_ != 0 && n_ >= 2 && bond0 < 0) {
         // snapshots of a recorded chunk stay resident until its entropies are
         // taken; bound the chunk to ~8 GiB of snapshots
         const std::size_t per = off_[n_] * sizeof(double2);
@@ -2196,7 +2237,87 @@ void Tebd::run(const double* a, const double* b, int count, int rec0) {
         }
         return;
     }
-    for (int k = 0; k < count; ++k) step(a[k], b[k], rec0 >= 0 ? rec0 + k : -1);
+    if (graph_on_ && !timing_ && plan_graphable_ && n_ >= 2 && count >= 2) {
+        run_graph(a, b, count, rec0, bond0);
+        return;
+    }
+    for (int k = 0; k < count; ++k) {
+        step(a[k], b[k], rec0 >= 0 ? rec0 + k : -1);
+        if (bond0 >= 0) record_max_bond(bond0 + k);
+    }
+}
+
+// Per-update path as a hipGraph: one Trotter step's launch sequence (single-site
+// layers, every two-site update of the routed ZZ block, the entropy sweep when
+// recording, and a node advancing the step index) is captured once and
+// replayed `count` times. Its kernels read the step's schedule values, ZZ gate
+// phasors and entropy slot from device memory (StepParams), computed on the host
+// with the same calls as the launch-per-update path, so both give the same bits.
+void Tebd::run_graph(const double* a, const double* b, int count, int rec0, int bond0) {
+    sync();  // the previous run's host staging below is free again
+    const int m = (int)model_.ew.size();
+    const double sub = dt_ / ns_;
+    g_host_ab_.resize(2 * (std::size_t)count);
+    g_host_zz_.resize((std::size_t)count * std::max(m, 1));
+    for (int k = 0; k < count; ++k) {
+        g_host_ab_[2 * k] = a[k];
+        g_host_ab_[2 * k + 1] = b[k];
+        for (int e = 0; e < m; ++e) {  // make_zz_gate(-B J dt / ns)
+            const double th = -b[k] * model_.ew[e] * sub;
+            g_host_zz_[(std::size_t)k * m + e] = make_double2(std::cos(th), std::sin(th));
+        }
+    }
+    if (g_ab_.n < g_host_ab_.size()) g_ab_.alloc(g_host_ab_.size());
+    if (g_zz_.n < g_host_zz_.size()) g_zz_.alloc(g_host_zz_.size());
+    if (!g_par_.p) g_par_.alloc(3);
+    g_host_par_[0] = 0;
+    g_host_par_[1] = rec0 >= 0 ? rec0 : 0;
+    g_host_par_[2] = bond0 >= 0 ? bond0 : 0;
+    TQ_HIP(hipMemcpyAsync(g_ab_.p, g_host_ab_.data(), g_host_ab_.size() * sizeof(double), hipMemcpyHostToDevice,
+                          stream_));
+    TQ_HIP(hipMemcpyAsync(g_zz_.p, g_host_zz_.data(), g_host_zz_.size() * sizeof(double2), hipMemcpyHostToDevice,
+                          stream_));
+    TQ_HIP(hipMemcpyAsync(g_par_.p, g_host_par_, 3 * sizeof(int), hipMemcpyHostToDevice, stream_));
+    const bool record = rec0 >= 0, bonds = bond0 >= 0;
+    const int mode = (int)record | ((int)bonds << 1);
+    const void* key[5] = {g_ab_.p, g_zz_.p, trace_.p, bias_.p, mbond_.p};
+    bool same = graph_exec_ && graph_rec_ == mode;
+    for (int q = 0; q < 5; ++q) same = same && graph_key_[q] == key[q];
+    if (!same) {
+        drop_graph();
+        sp_ = StepParams{g_ab_.p, g_zz_.p, g_par_.p, m};
+        capturing_ = true;
+        TQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            step(0.0, 0.0, record ? 0 : -1);  // A, B and the ZZ gates come from StepParams
+            if (bonds) record_max_bond(0);
+            hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, stream_, g_par_.p);
+            TQ_LAUNCH_CHECK();
+        } catch (...) {
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            capturing_ = false;
+            sp_ = StepParams{};
+            throw;
+        }
+        capturing_ = false;
+        sp_ = StepParams{};
+        TQ_HIP(hipStreamEndCapture(stream_, &g));
+        graph_ = g;
+        TQ_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        graph_rec_ = mode;
+        for (int q = 0; q < 5; ++q) graph_key_[q] = key[q];
+    }
+    for (int k = 0; k < count; ++k) TQ_HIP(hipGraphLaunch(graph_exec_, stream_));
+}
+
+void Tebd::drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    graph_rec_ = -1;
 }
 
 void Tebd::set_executor(int groups, int workers) {
@@ -2709,7 +2830,7 @@ void Tebd::entropy_into(const double2* sites, const int* dims, double* dst) {
         std::swap(cur, nxt);
     }
     hipLaunchKernelGGL(k_rho0, dim3(1), dim3(64), 0, stream_, sites, off_dev_.p, caps_dev_.p, dims, cur,
-                       chi_max(), dst, 0);
+                       chi_max(), dst, 0, capturing_ ? sp_.par : nullptr);
     TQ_LAUNCH_CHECK();
 }
 
@@ -2890,7 +3011,7 @@ std::vector<double> Tebd::theta(int p, int& rows, int& cols) {
     cols = 2 * Dr;
     const long long ttiles = theta_tiles_for(cap_[p], cap_[p + 2]);
     hipLaunchKernelGGL(k_theta, dim3((unsigned)((ttiles + 3) / 4)), dim3(256), 0, stream_, sites_.p, off_dev_.p,
-                       caps_dev_.p, dims_.p, p, eye, mat_.p, work_.p, ld_);
+                       caps_dev_.p, dims_.p, p, eye, mat_.p, work_.p, ld_, StepParams{}, -1);
     TQ_LAUNCH_CHECK();
     std::vector<double2> m((std::size_t)ld_ * ld_);
     TQ_HIP(hipMemcpyAsync(m.data(), mat_.p, m.size() * sizeof(double2), hipMemcpyDeviceToHost, stream_));

@@ -29,6 +29,18 @@ struct SvdStats {
     long long supersteps = 0, exec_updates = 0;  // lockstep executor: super-steps run, updates placed
 };
 
+// A Trotter step's values on the device, for a step captured once and replayed
+// (hipGraph): schedule values (a[k], b[k]) at ab[2k], ZZ gate phasors (cos, sin
+// of -b[k] J_e dt / ns) at zz[k * m + e], par = {k, first recorded entropy slot}.
+// par[2] = first max-bond slot. All null outside a capture: kernels then take
+// their launch arguments.
+struct StepParams {
+    const double* ab = nullptr;
+    const double2* zz = nullptr;
+    const int* par = nullptr;
+    int m = 0;
+};
+
 // One two-site update of the routed ZZ block.
 struct Update {
     int site;   // acts on (site, site+1)
@@ -53,7 +65,9 @@ public:
     // Enqueue `count` steps with schedule values a[k], b[k]; when rec0 >= 0 the
     // entropy of step k is recorded at slot rec0 + k. Runs the steps one launch
     // sequence per update, or as one persistent launch when the executor is on.
-    void run(const double* a, const double* b, int count, int rec0);
+    // bond0 >= 0: the largest bond after step k is kept at max-bond slot bond0 + k
+    // (read with max_bonds after sync; the steps then run on the per-update path).
+    void run(const double* a, const double* b, int count, int rec0, int bond0 = -1);
     // Lockstep executor: 0 = off, g > 0 = at most g groups of workgroups,
     // -1 = as many groups as are co-resident (capped at 64). workers: workgroups
     // per group (0 = one per block pair of a tournament round).
@@ -108,7 +122,9 @@ public:
 private:
     void build_plan();
     void run_exec(const double* a, const double* b, int count, int rec0);
-    void two_site(int p, const Gate4& g, const double* wts = nullptr);
+    void two_site(int p, const Gate4& g, const double* wts = nullptr, int edge = -1);
+    void run_graph(const double* a, const double* b, int count, int rec0, int bond0);
+    void drop_graph();
     void entropy_sweep(int rec);
     void entropy_of(const double2* sites, const int* dims, int rec);
     void entropy_into(const double2* sites, const int* dims, double* dst);
@@ -177,6 +193,22 @@ private:
     bool ex_dataflow_ = false;             // per-group layout: updates wait for their predecessors only
     DevBuf<int> ex_done_;                  // their done flags
     std::vector<unsigned char> ex_host_;
+
+    // per-update path captured as a hipGraph (run_graph)
+    bool graph_on_ = true;         // knob TESSERA_STEP_GRAPH=0 turns it off
+    bool plan_graphable_ = false;  // no update needs the cooperative SVD launch
+    bool capturing_ = false;
+    StepParams sp_;                // device step values while capturing
+    DevBuf<double> g_ab_;
+    DevBuf<double2> g_zz_;
+    DevBuf<int> g_par_;
+    std::vector<double> g_host_ab_;
+    std::vector<double2> g_host_zz_;
+    int g_host_par_[3] = {0, 0, 0};
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    int graph_rec_ = -1;
+    const void* graph_key_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 }  // namespace tq

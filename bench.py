@@ -314,13 +314,17 @@ def batch_bench(args):
     t_solo = sum(solo_times)
     progress(f"one at a time: {len(insts)} instances x {args.steps} steps in {t_solo:.2f}s")
     mode = args.exec_mode
+    rec = None
     if mode == -1:
         from tessera_amd import tuning
 
         rec = tuning.tune(max(x.chi for x in insts), len(insts))
         mode = rec["mode"]
-        progress(f"layout tuner: {rec}")
+        progress(f"tuner: {rec}")
     ss = sessions()
+    if rec is not None:  # batched runs take the first session's executor shape and SVD option
+        ss[0].set_executor(0, rec["workers"])
+        ss[0].set_svd_lq(rec["lq"])
     t0 = time.perf_counter()
     qa.batch_run(ss, [sched(x) for x in insts], args.executor or -1, mode)
     for s in ss:
@@ -350,7 +354,7 @@ def batch_bench(args):
                    "batched_instance_s_p50": float(np.percentile(batched_ms, 50)) * 1e-3,
                    "batched_instance_s_p99": float(np.percentile(batched_ms, 99)) * 1e-3,
                    "batched_launch_s": t_batch,
-                   "exec_mode": mode, "supersteps": ex["supersteps"],
+                   "exec_mode": mode, "tuner": rec, "supersteps": ex["supersteps"],
                    "updates_per_superstep": ex["updates"] / max(ex["supersteps"], 1)},
     }
     print(json.dumps(out), flush=True)

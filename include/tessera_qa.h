@@ -202,6 +202,11 @@ int tessera_tebd_set_executor(tessera_tebd* t, int groups);
  * each taking several pairs per round (fewer workgroups in every barrier). In a
  * batched run the first session's setting applies. */
 int tessera_tebd_set_executor_shape(tessera_tebd* t, int groups, int workers);
+/* Householder LQ preconditioning of the multi-workgroup Jacobi SVD (on != 0, the
+ * default) or Jacobi directly on theta (0): the same factorisation, a different
+ * sweep count; a knob of the autotuner. In a batched run the first session's
+ * setting applies. */
+int tessera_tebd_set_svd_lq(tessera_tebd* t, int on);
 /* Batched anneal (no upstream counterpart; the batched scheduler): advance
  * count sessions together, session i by steps[i] Trotter steps with schedule
  * values a[i][k], b[i][k], in one cooperative launch whose super-steps take

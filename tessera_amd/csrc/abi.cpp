@@ -474,6 +474,17 @@ int tessera_tebd_set_executor(tessera_tebd* t, int groups) {
     return tessera_tebd_set_executor_shape(t, groups, 0);
 }
 
+int tessera_tebd_set_svd_lq(tessera_tebd* t, int on) {
+    try {
+        if (!t) throw Error(TESSERA_ERR_ARG, "null session");
+        t->eng->set_svd_lq(on != 0);
+        return TESSERA_OK;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return status_of(e);
+    }
+}
+
 int tessera_tebd_set_executor_shape(tessera_tebd* t, int groups, int workers) {
     try {
         if (!t) throw Error(TESSERA_ERR_ARG, "null session");

@@ -1792,7 +1792,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 } else {
                     SvdMode mode;  // own tournament length, own convergence
                     svd_blocks(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, 0, 1, bs, lds,
-                               sh, mode, a.aux + grp * mat_sz, a.taus + (std::size_t)grp * a.ld);
+                               sh, mode, (a.aux ? a.aux + grp * mat_sz : nullptr), a.taus + (std::size_t)grp * a.ld);
                 }
                 bs();
                 recon_part(I.sites, I.off, I.cap, I.dims, u.site, M, W, a.ld, sv, perm, tid, blockDim.x);
@@ -1824,7 +1824,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 mode.err = a.err;
                 mode.spin_limit = a.spin_limit;
                 svd_blocks<GroupSync<XL>, XL>(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, rank,
-                                              nwg, gs, lds, sh, mode, a.aux + grp * mat_sz,
+                                              nwg, gs, lds, sh, mode, (a.aux ? a.aux + grp * mat_sz : nullptr),
                                               a.taus + (std::size_t)grp * a.ld);
                 mark(3);
                 gs();
@@ -1858,7 +1858,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
             mode.lq_panels = (a.ld + 15) / 16;
             mode.prof = blockIdx.x == 0;
             svd_blocks(W, a.ld, I.dims, active ? u.site : 0, I.chi, sv, perm, I.trunc, sflags, I.stats, rank, nwg, sync,
-                       lds, sh, mode, a.aux + grp * mat_sz, a.taus + (std::size_t)grp * a.ld);
+                       lds, sh, mode, (a.aux ? a.aux + grp * mat_sz : nullptr), a.taus + (std::size_t)grp * a.ld);
             mark(3);
             sync();
             mark(4);
@@ -2161,9 +2161,10 @@ void Tebd::two_site(int p, const Gate4& g, const double* wts, int edge) {
         int pp = p, cc = chi_;
         unsigned long long* stats = stats_.p;
         int xs = xcd_stride_;
+        double2* lqa = svd_lq_ ? aux_.p : nullptr;  // no reflector buffer: no LQ preconditioning
         void* args[] = {(void*)&work_.p, (void*)&ld_,   (void*)&dims_.p,  (void*)&pp,   (void*)&cc,
                         (void*)&sv_.p,   (void*)&perm_.p, (void*)&trunc_.p, (void*)&flags, (void*)&stats,
-                        (void*)&xs,      (void*)&aux_.p,  (void*)&taus_.p};
+                        (void*)&xs,      (void*)&lqa,     (void*)&taus_.p};
         TQ_HIP(hipLaunchCooperativeKernel((const void*)k_svd_dist, dim3(nwg), dim3(kDistThreads), args, shm, stream_));
     }
     if (timing_) TQ_HIP(hipEventRecord(events_[ev_next_++].second, stream_));
@@ -2318,6 +2319,11 @@ void Tebd::drop_graph() {
     graph_exec_ = nullptr;
     graph_ = nullptr;
     graph_rec_ = -1;
+}
+
+void Tebd::set_svd_lq(bool on) {
+    sync();
+    svd_lq_ = on;
 }
 
 void Tebd::set_executor(int groups, int workers) {
@@ -2682,7 +2688,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.snap = t0.ex_snap_.p;
     x.snap_stride = t0.off_[t0.n_];
     x.snap_dims = t0.ex_sdims_.p;
-    x.aux = t0.ex_aux_.p;
+    x.aux = t0.svd_lq_ ? t0.ex_aux_.p : nullptr;  // null: Jacobi on theta itself, no LQ preconditioning
     x.taus = t0.ex_taus_.p;
     x.err = t0.ex_bar_.p;  // the error word, then [G][2] barrier words
     x.spin_limit = t0.ex_spin_limit_;

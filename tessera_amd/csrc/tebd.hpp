@@ -73,6 +73,11 @@ public:
     // per group (0 = one per block pair of a tournament round).
     void set_executor(int groups, int workers = 0);
     int executor_groups() const { return exec_req_; }
+    int executor_workers() const { return exec_workers_; }
+    // LQ preconditioning of the multi-workgroup Jacobi SVD (on by default); in a
+    // batched run the first session's setting applies.
+    void set_svd_lq(bool on);
+    bool svd_lq() const { return svd_lq_; }
     // Batched lockstep run: instance i advances counts[i] steps with schedule
     // values as[i][k], bs[i][k]; entropy recording (rec0 >= 0) needs one instance.
     // mode: 0 auto (solo for four or more instances with chi <= 32), 1 grouped
@@ -173,6 +178,7 @@ private:
     std::size_t svd_lds_ = 0;
 
     // lockstep executor state
+    bool svd_lq_ = true;
     int exec_req_ = 0;
     int exec_workers_ = 0;
     long long exec_supersteps_ = 0, exec_updates_ = 0;

@@ -87,6 +87,8 @@ def lib() -> ctypes.CDLL:
         L.tessera_tebd_set_timing.argtypes = [vp, ctypes.c_int]
         L.tessera_tebd_set_executor.restype = ctypes.c_int
         L.tessera_tebd_set_executor.argtypes = [vp, ctypes.c_int]
+        L.tessera_tebd_set_svd_lq.restype = ctypes.c_int
+        L.tessera_tebd_set_svd_lq.argtypes = [vp, ctypes.c_int]
         L.tessera_tebd_set_executor_shape.restype = ctypes.c_int
         L.tessera_tebd_set_executor_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.tessera_tebd_theta.restype = ctypes.c_int
@@ -427,6 +429,10 @@ class TebdSession:
         per run() with at most g concurrent groups of workgroups; -1: as many as fit.
         workers > 0 caps the workgroups per group (each takes several block pairs)."""
         _check(lib().tessera_tebd_set_executor_shape(self._h, int(groups), int(workers)))
+
+    def set_svd_lq(self, on: bool = True):
+        """LQ preconditioning of the multi-workgroup Jacobi SVD (on by default)."""
+        _check(lib().tessera_tebd_set_svd_lq(self._h, 1 if on else 0))
 
     def theta(self, p: int) -> np.ndarray:
         """Identity-gate theta of sites (p, p+1): the (2Dl, 2Dr) matrix the next

@@ -126,14 +126,18 @@ def test_batch_config4_sample_runs():
 
 
 def test_layout_tuner_picks_a_measured_layout_and_runs_it(tmp_path):
-    """The tuner measures both layouts on saturated probe chains, caches the
-    faster (also in a JSON file) and the tuned batch equals a direct batch in
-    that layout."""
+    """The tuner measures every candidate configuration (solo layout; grouped with
+    one or half a workgroup per block pair; with and without LQ preconditioning)
+    on saturated probe chains, caches the fastest (also in a JSON file) and the
+    tuned batch equals a direct batch in that configuration."""
     from tessera_amd import tuning
 
     cache = tmp_path / "layouts.json"
     rec = tuning.tune(16, 6, cache_path=cache)
-    assert rec["mode"] in (1, 2) and set(rec["updates_per_s"]) == {"grouped", "solo"}
+    names = {c[0] for c in tuning._configs(16)}
+    print(f"tuner record: {rec}", flush=True)
+    assert rec["mode"] in (1, 2) and set(rec["updates_per_s"]) == names and rec["config"] in names
+    assert {"solo", "grouped_w2_lq", "grouped_w2_nolq"} <= names
     assert all(v > 0 for v in rec["updates_per_s"].values())
     assert cache.exists() and tuning.tune(16, 5, cache_path=cache) is rec  # same class, cached
     cases = _mixed()[:3]
@@ -143,7 +147,10 @@ def test_layout_tuner_picks_a_measured_layout_and_runs_it(tmp_path):
         if tuned:
             tuning.tuned_batch_run(sessions, [c[1] for c in cases], [c[3] for c in cases])
         else:
-            qa.batch_run(sessions, [c[1] for c in cases], -1, tuning.tune(64, 3)["mode"])
+            r = tuning.tune(64, 3)
+            sessions[0].set_executor(0, r["workers"])
+            sessions[0].set_svd_lq(r["lq"])
+            qa.batch_run(sessions, [c[1] for c in cases], -1, r["mode"])
         out.append([s.dense() for s in sessions])
         for s in sessions:
             s.close()

@@ -199,6 +199,38 @@ def test_xcd_local_groups_match_device_scope_barriers(monkeypatch, chi, workers,
     assert xl["max_trunc"] == dev["max_trunc"]
 
 
+@pytest.mark.parametrize("groups", [8, 0])
+def test_svd_without_lq_preconditioning_matches(monkeypatch, groups):
+    """The tuner's LQ switch: the multi-workgroup Jacobi SVD straight on theta
+    (tessera_tebd_set_svd_lq(0)) factors the same matrices with another sweep
+    count, so at exact chi (chi = 64 on this 12-site chain: discarded weight
+    ~1e-30) the run equals the preconditioned one to rounding; in the executor
+    (groups = 8) and on the per-update path forced onto the cooperative SVD
+    (groups = 0). (At chi = 32 the run truncates 8e-3 of the weight and the two
+    factorisations' rounding is amplified, as between any two accurate SVDs.)"""
+    if groups == 0:
+        monkeypatch.setenv("TESSERA_FORCE_DIST_SVD", "1")
+    model = I.chain_range(12, 3, 23)
+    sched = I.linear(5)
+    out = []
+    for lq in (True, False):
+        s = qa.TebdSession(model, 64, 0.1)
+        s.set_executor(groups)
+        s.set_svd_lq(lq)
+        s.run(sched.a, sched.b)
+        ent, tr, bonds = s.sync()
+        st = s.svd_stats()
+        out.append((s.dense(), ent, tr, st["sweeps"] / max(st["calls"], 1)))
+        s.close()
+    (p1, e1, t1, w1), (p0, e0, t0, w0) = out
+    inf = 1 - _fid(p1, p0)
+    print(f"groups={groups}: LQ on vs off infidelity {inf:.2e}, entropy {np.abs(e1 - e0).max():.2e}, "
+          f"sweeps per SVD {w1:.2f} / {w0:.2f}", flush=True)
+    assert inf <= 1e-12
+    assert np.abs(e1 - e0).max() <= 1e-10
+    assert t1 <= 1e-20 and t0 <= 1e-20
+
+
 def test_exec_profile_phases_are_consistent():
     """tessera_tebd_exec_profile: workgroup 0's phase timers partition each launch
     (snapshots + theta, SVD, wait after it, reconstruction, super-step-end wait),

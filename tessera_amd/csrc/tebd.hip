@@ -461,7 +461,7 @@ constexpr int kStatSlots = 1024;   // per-workgroup accounting slots (largest gr
 // Shared LDS of the block-Jacobi SVD (static part).
 // Executor phase profile of workgroup 0, in wall-clock ticks (slot meanings
 // at k_tebd_lockstep).
-constexpr int kProfSlots = 12;
+constexpr int kProfSlots = 17;
 
 struct SvdShared {
     unsigned long long tm[kProfSlots];  // phase profile (workgroup 0, thread 0)
@@ -687,6 +687,7 @@ struct SvdMode {
     int* epochs = nullptr;
     int* err = nullptr;         // raised by a round wait that gave up
     long long spin_limit = 0;   // polls before a round wait gives up
+    int naps = 1;               // 64-cycle sleeps between polls
 };
 
 // Worker holding block b in tournament round r of rr_pair with one pair per
@@ -702,8 +703,10 @@ __device__ inline int rr_holder(int r, int b, int cp) {
 // Thread 0 of a workgroup waits until a round counter reaches `target`; every
 // wait ends (counter reached, another wait gave up, or the poll limit, which
 // raises *err).
+// Between polls the thread sleeps `naps` times 64 cycles: each device-scope
+// acquire poll invalidates the XCD's L2 lines that other workgroups are reading.
 template <bool XL = false>
-__device__ inline void wait_counter(int* ctr, int target, int* err, long long limit) {
+__device__ inline void wait_counter(int* ctr, int target, int* err, long long limit, int naps = 1) {
     // XCD-local: the poll reads the L2 (sc1) and acquires nothing; the caller
     // reads the producer's data with L1-bypassing loads
     constexpr int order = XL ? __ATOMIC_RELAXED : __ATOMIC_ACQUIRE;
@@ -712,7 +715,7 @@ __device__ inline void wait_counter(int* ctr, int target, int* err, long long li
             __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return;
         }
-        __builtin_amdgcn_s_sleep(1);
+        for (int q = 0; q < naps; ++q) __builtin_amdgcn_s_sleep(1);
     }
 }
 
@@ -789,8 +792,10 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                     int bi, bj;
                     rr_pair(r, me, nbp, bi, bj);
                     const int pa = rr_holder(r - 1, bi, nbp), pb = rr_holder(r - 1, bj, nbp);
-                    if (pa != me) wait_counter<XL>(&mode.epochs[(std::size_t)pa * 64], ep, mode.err, mode.spin_limit);
-                    if (pb != me) wait_counter<XL>(&mode.epochs[(std::size_t)pb * 64], ep, mode.err, mode.spin_limit);
+                    if (pa != me)
+                        wait_counter<XL>(&mode.epochs[(std::size_t)pa * 64], ep, mode.err, mode.spin_limit, mode.naps);
+                    if (pb != me)
+                        wait_counter<XL>(&mode.epochs[(std::size_t)pb * 64], ep, mode.err, mode.spin_limit, mode.naps);
                     // device scope: the producers' blocks, after their release; XCD-local:
                     // the blocks are read with L1-bypassing loads below
                     if constexpr (!XL) __threadfence();
@@ -809,6 +814,14 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 // LDS column q <-> block (q < 8 ? bi : bj), member q & 7; a column's
                 // rows go through registers with all of a lane's loads in flight at
                 // once (a load-store loop would wait out each load in turn)
+                unsigned long long tr0 = prof ? wall_clock64() : 0;
+                auto lap = [&](int slot) {  // round phases of workgroup 0
+                    if (prof) {
+                        const unsigned long long t = wall_clock64();
+                        sh.tm[slot] += t - tr0;
+                        tr0 = t;
+                    }
+                };
                 for (int q = wave; q < 2 * kBlk; q += kDistThreads / 64) {
                     const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
                     if (col < c) {
@@ -817,6 +830,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                     }
                 }
                 __syncthreads();
+                lap(12);
                 bool rotated = false;
                 if (r == 0) {  // pairs inside each block, once per sweep
                     for (int s = 0; s < kBlk - 1; ++s) {
@@ -840,6 +854,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                         __syncthreads();
                     }
                 }
+                lap(13);
                 // cross pairs with carried norms
                 const bool mine = bi * kBlk + wave < c;
                 double al = 0.0;
@@ -872,6 +887,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 if (mine) store_col(xa, lds + (std::size_t)wave * m, m, lane);
                 if (rotated && lane == 0) sh.rot = 1;
                 __syncthreads();
+                lap(14);
                 if (tid == 0 && sh.rot) {
                     atomicOr(&flags[sweep & 1], 1);
                     if (mode.gflags) atomicOr(&mode.gflags[sweep & 1], 1);
@@ -884,8 +900,10 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                     }
                 }
                 __syncthreads();  // the next pair of this worker reuses the LDS columns
+                lap(15);
             }
             if (p2p && r < rounds - 1) {
+                const unsigned long long tpub = prof ? wall_clock64() : 0;
                 // publish this round's blocks: every wave waits until its write-back
                 // stores are complete in the L2 (s_waitcnt of all counters; stores
                 // count in vmcnt on CDNA), the workgroup barrier follows, and thread 0's
@@ -904,7 +922,10 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                                            __HIP_MEMORY_SCOPE_AGENT);
                     }
                 }
-                if (prof) sh.tm[9] += 1;
+                if (prof) {
+                    sh.tm[9] += 1;
+                    sh.tm[16] += wall_clock64() - tpub;
+                }
             } else {
                 const unsigned long long tw = prof ? wall_clock64() : 0;
                 sync();
@@ -1623,6 +1644,7 @@ struct ExecArgs {
     int* group_bar;     // [G][2] arrival count, generation
     int* err;           // raised by a GroupSync wait that gave up
     long long spin_limit;  // GroupSync polls before giving up
+    int naps;              // 64-cycle sleeps between the polls of a wait
     int* epochs;        // [G][W][64] round counters of the point-to-point rounds (per-group layout)
     int dataflow;       // per-group layout without snapshots: no super-step barriers; an update
                         // waits for the done flags of its predecessors instead
@@ -1644,6 +1666,7 @@ struct GroupSync {
     int nwg;
     int* err;
     long long limit;  // polls before giving up (kSpinLimit unless a test lowers it)
+    int naps = 1;     // 64-cycle sleeps between polls
     __device__ void operator()() const {
         __builtin_amdgcn_s_waitcnt(0);  // every wave's stores complete in the L2 before the arrival
         __syncthreads();
@@ -1677,7 +1700,7 @@ struct GroupSync {
                             __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
                         }
-                        __builtin_amdgcn_s_sleep(2);
+                        for (int q = 0; q < 2 * naps; ++q) __builtin_amdgcn_s_sleep(1);
                     }
                 }
                 __threadfence();  // the other workgroups' writes, after the release
@@ -1815,7 +1838,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 }
                 // theta, SVD and reconstruction of this group's update on the
                 // group's own barrier: own tournament length and convergence
-                GroupSync<XL> gs{a.group_bar + 2 * grp, nwg, a.err, a.spin_limit};
+                GroupSync<XL> gs{a.group_bar + 2 * grp, nwg, a.err, a.spin_limit, a.naps};
                 theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg);
                 mark(0);
                 SvdMode mode;
@@ -1823,6 +1846,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 mode.epochs = a.epochs ? a.epochs + (std::size_t)grp * nwg * 64 : nullptr;
                 mode.err = a.err;
                 mode.spin_limit = a.spin_limit;
+                mode.naps = a.naps;
                 svd_blocks<GroupSync<XL>, XL>(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, rank,
                                               nwg, gs, lds, sh, mode, (a.aux ? a.aux + grp * mat_sz : nullptr),
                                               a.taus + (std::size_t)grp * a.ld);
@@ -1981,6 +2005,8 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     const char* df = std::getenv("TESSERA_EXEC_DATAFLOW");
     ex_dataflow_ = df && df[0] == '1';
     // test knob: polls before a group barrier wait gives up (forces the error path)
+    const char* nap = std::getenv("TESSERA_EXEC_POLL_NAPS");  // knob: 64-cycle sleeps between polls
+    ex_naps_ = nap ? std::max(1, std::atoi(nap)) : 1;
     const char* sl = std::getenv("TESSERA_EXEC_SPIN_LIMIT");
     ex_spin_limit_ = sl ? std::max(0ll, std::atoll(sl)) : kSpinLimit;
     // knob, off by default: 1 synchronises each group of the grouped executor
@@ -2692,6 +2718,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.taus = t0.ex_taus_.p;
     x.err = t0.ex_bar_.p;  // the error word, then [G][2] barrier words
     x.spin_limit = t0.ex_spin_limit_;
+    x.naps = t0.ex_naps_;
     x.dataflow = (group_sync && t0.ex_dataflow_ && snap_list.empty()) ? 1 : 0;
     x.done = nullptr;
     if (x.dataflow) {

@@ -194,6 +194,7 @@ private:
     bool ex_xcd_local_ = false;            // ... XCD-local when the groups sit one per XCD (opt-in)
     bool ex_armed_ = false;                // a launch whose barrier error word is still unread
     long long ex_spin_limit_ = 1ll << 24;  // GroupSync polls before a wait gives up
+    int ex_naps_ = 1;                      // 64-cycle sleeps between the polls of a wait
     bool ex_p2p_ = true;                   // per-group layout: point-to-point tournament rounds
     DevBuf<int> ex_ep_;                    // their round counters
     bool ex_dataflow_ = false;             // per-group layout: updates wait for their predecessors only

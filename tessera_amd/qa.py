@@ -461,7 +461,9 @@ class TebdSession:
         return dict(supersteps=ss.value, updates=up.value)
 
     PROFILE_SLOTS = ("theta_ms", "lq_ms", "jacobi_ms", "svd_ms", "post_svd_wait_ms", "recon_ms",
-                     "superstep_wait_ms", "active_supersteps", "round_barrier_ms", "rounds", "kernel_ms", "sweeps")
+                     "superstep_wait_ms", "active_supersteps", "round_barrier_ms", "rounds", "kernel_ms", "sweeps",
+                     "round_load_ms", "round_inblock_ms", "round_cross_ms", "round_writeback_ms",
+                     "round_publish_ms")
 
     def exec_profile(self) -> dict:
         """Phase profile of the lockstep executor's first workgroup since creation

@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--epochs", type=int, default=60, help="learned workload: training epochs")
     ap.add_argument("--fractions", default="0.25,0.5", help="learned workload: shortened anneal lengths")
     ap.add_argument("--strengths", default="0.5,1.0", help="learned workload: warm-start bias strengths")
+    ap.add_argument("--step-profile", action="store_true",
+                    help="tebd: add the executor's cycle profile of the Jacobi rotation steps (slows workgroup 0)")
     ap.add_argument("--exec-workers", type=int, default=0,
                     help="lockstep executor: workgroups per group (0: one per block pair of a round)")
     ap.add_argument("--instances", type=int, default=32, help="batch workload: number of batched-workload instances")
@@ -552,6 +554,24 @@ def pmc_traffic(kernel: str, updates: int):
     return per * updates, rec
 
 
+def sweep_summary(h0, h1, rounds):
+    """Per Jacobi sweep index: SVDs of workgroup 0 that ran it and its rotations per
+    SVD there (the executor's convergence profile over the timed steps), and its
+    thread 0's shader cycles per tournament round in the parts of the row-layout
+    rotation steps."""
+    import numpy as np
+
+    svds = h1["svds"] - h0["svds"]
+    rot = h1["rotations"] - h0["rotations"]
+    last = int(np.flatnonzero(svds)[-1]) + 1 if svds.any() else 0
+    cyc = {k: round((h1["step_cycles"][k] - h0["step_cycles"][k]) / max(rounds, 1), 1) for k in h1["step_cycles"]}
+    out = {"svds": [int(x) for x in svds[:last]],
+           "rotations_per_svd": [round(float(r / v), 1) if v else 0.0 for r, v in zip(rot[:last], svds[:last])]}
+    if any(cyc.values()):
+        out["step_cycles_per_round"] = cyc
+    return out
+
+
 def tebd_timed(args, model, sched, k0, sess):
     """The headline measurement on an open session (closed by the caller)."""
     sess.set_executor(args.executor, args.exec_workers)
@@ -569,6 +589,7 @@ def tebd_timed(args, model, sched, k0, sess):
     st0 = sess.svd_stats()
     ex0 = sess.exec_stats()
     pr0 = sess.exec_profile()
+    sw0 = sess.sweep_profile()
     t0 = time.perf_counter()
     # the timed steps are the anneal's own: single-site layers, routed ZZ block and
     # the qubit-0 entropy of every step (recorded, as mps.cpp:355 does)
@@ -619,7 +640,9 @@ def tebd_timed(args, model, sched, k0, sess):
                   "group_barriers": os.environ.get("TESSERA_EXEC_LOCKSTEP", "0") != "1",
                   "point_to_point_rounds": (os.environ.get("TESSERA_EXEC_LOCKSTEP", "0") != "1"
                                             and os.environ.get("TESSERA_EXEC_P2P", "1") != "0"),
-                  "workgroup0_phases": {k: round(pr1[k] - pr0[k], 3) for k in pr1}}
+                  "workgroup0_phases": {k: round(pr1[k] - pr0[k], 3) for k in pr1},
+                  "workgroup0_sweeps": sweep_summary(sw0, sess.sweep_profile(), pr1["rounds"] - pr0["rounds"]),
+                  "svd_stop_ratio": float(os.environ.get("TESSERA_SVD_STOP", "0") or 0)}
     else:
         kname = "k_svd_dist"
         kernel = "per-update launches; timed: the SVD launches (k_svd_dist / k_svd)"
@@ -659,6 +682,8 @@ def tebd_timed(args, model, sched, k0, sess):
 
 def main():
     args = parse()
+    if getattr(args, "step_profile", False):
+        os.environ["TESSERA_EXEC_STEP_PROFILE"] = "1"  # read when the session is created
     if args.gpus != 1:
         print(json.dumps({"error": "this implementation runs on one GPU; multi-GPU sharding is designed, not built",
                           "n_gpus": args.gpus}))

@@ -274,12 +274,16 @@ constexpr double kNoiseAbs = 5e-15;
 __device__ inline double noise_floor2(double fro2) { return fmin(kNoise * kNoise * fro2, kNoiseAbs * kNoiseAbs); }
 
 // One Jacobi rotation of columns x, y (lane-held rows); returns true if applied.
-__device__ inline bool pair_rotate(double2 (&x)[kRowsPerLane], double2 (&y)[kRowsPerLane], int m, double floor2) {
+// `big` is raised when the pair's coupling |x^H y| / (|x| |y|) exceeds big_tol.
+__device__ inline bool pair_rotate(double2 (&x)[kRowsPerLane], double2 (&y)[kRowsPerLane], int m, double floor2,
+                                   double big_tol = 0.0, bool* big = nullptr) {
     double al, be, gr, gi;
     pair_dots(x, y, al, be, gr, gi);
     if (al <= floor2 || be <= floor2) return false;
     const double gabs = sqrt(gr * gr + gi * gi);
-    if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * sqrt(al * be)) return false;
+    const double scale = sqrt(al * be);
+    if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * scale) return false;
+    if (big && gabs > big_tol * scale) *big = true;
     const double2 ph = make_double2(gr / gabs, -gi / gabs);  // conj(gamma)/|gamma|
     const double zeta = (be - al) / (2.0 * gabs);
     const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
@@ -461,7 +465,10 @@ constexpr int kStatSlots = 1024;   // per-workgroup accounting slots (largest gr
 // Shared LDS of the block-Jacobi SVD (static part).
 // Executor phase profile of workgroup 0, in wall-clock ticks (slot meanings
 // at k_tebd_lockstep).
-constexpr int kProfSlots = 17;
+constexpr int kHistSweeps = 24;  // sweep histogram: sweeps past the last fold into it
+constexpr int kProfHist = 17;    // first histogram slot
+constexpr int kProfStep = kProfHist + 3 * kHistSweeps;  // row-layout step parts (cycles)
+constexpr int kProfSlots = kProfStep + 5;
 
 struct SvdShared {
     unsigned long long tm[kProfSlots];  // phase profile (workgroup 0, thread 0)
@@ -637,7 +644,7 @@ __device__ inline double col_norm2(const double2 (&x)[kRowsPerLane]) {
 // pair_rotate with carried norms^2 al (x) and be (y): only g = x^H y is
 // reduced; the norms are updated (recomputed when they shrink by 64x or more).
 __device__ inline bool pair_rotate_carried(double2 (&x)[kRowsPerLane], double2 (&y)[kRowsPerLane], int m,
-                                           double& al, double& be, double floor2) {
+                                           double& al, double& be, double floor2, double big_tol, bool& big) {
     if (al <= floor2 || be <= floor2) return false;
     double gr = 0.0, gi = 0.0;
 #pragma unroll
@@ -652,7 +659,9 @@ __device__ inline bool pair_rotate_carried(double2 (&x)[kRowsPerLane], double2 (
         gi += __shfl_xor(gi, off, 64);
     }
     const double gabs = sqrt(gr * gr + gi * gi);
-    if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * sqrt(al * be)) return false;
+    const double scale = sqrt(al * be);
+    if (!(gabs > 0.0) || gabs <= jacobi_tol(m) * scale) return false;
+    if (gabs > big_tol * scale) big = true;
     const double2 ph = make_double2(gr / gabs, -gi / gabs);  // conj(gamma)/|gamma|
     const double zeta = (be - al) / (2.0 * gabs);
     const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
@@ -667,6 +676,282 @@ __device__ inline bool pair_rotate_carried(double2 (&x)[kRowsPerLane], double2 (
     al = (al1 < al * (1.0 / 64)) ? col_norm2(x) : al1;
     be = (be1 < be * (1.0 / 64)) ? col_norm2(y) : be1;
     return true;
+}
+
+// ---- row layout of a tournament round -------------------------------------
+// Thread t of the workgroup holds row t (t < m) of the 16 columns of the
+// round's block pair in registers: v[q], q < 8 the columns of block bi, q >= 8
+// those of block bj. A step rotates 8 disjoint column pairs: every lane forms
+// its row's share of the 8 products x^H y, one transposed wave reduction leaves
+// each lane one of the 16 partial sums, the waves' partials meet in LDS behind
+// one barrier, and every wave turns the totals into the 8 rotations (one pair
+// per lane, computed identically by each wave, so no second barrier) and
+// applies them to its own rows. The columns never pass through LDS. Column
+// norms^2 are exact at the start of the round and carried through its steps
+// (recomputed when one shrinks by 64x or more), as in pair_rotate_carried.
+struct RowLds {
+    double red[2][16][kDistThreads / 64];  // partial sums [parity][value][wave]
+    double rn[kDistThreads / 64][16];      // each wave's copy of the carried norms^2
+    double par[kDistThreads / 64][8][4];   // each wave's rotations: cs, sn, conj(g)/|g|
+};
+
+// Total over the waves of partial value (lane & 15) of buffer p: 8 contiguous
+// doubles per lane, read together.
+__device__ __forceinline__ double row_total(const RowLds& rl, int p, int lane) {
+    const double* r = rl.red[p][lane & 15];
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kDistThreads / 64; ++w) t += r[w];
+    return t;
+}
+
+// Cross-lane moves of doubles without LDS. swap32: lanes 32-63 of a trade with
+// lanes 0-31 of b (v_permlane32_swap); swap16: in each half, lanes 16-31 of a
+// trade with lanes 0-15 of b (v_permlane16_swap); dpp_f64<CTRL>: the value of
+// the lane the DPP control names (0x140 row_mirror: l ^ 15 within a row of 16,
+// 0x141 row_half_mirror: l ^ 7 within 8, 0x4E / 0xB1 quad_perm: l ^ 2 / l ^ 1).
+__device__ __forceinline__ void swap32(double& a, double& b) {
+    const long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+    const auto rl = __builtin_amdgcn_permlane32_swap((unsigned)x, (unsigned)y, false, false);
+    const auto rh = __builtin_amdgcn_permlane32_swap((unsigned)(x >> 32), (unsigned)(y >> 32), false, false);
+    a = __longlong_as_double(((long long)rh[0] << 32) | rl[0]);
+    b = __longlong_as_double(((long long)rh[1] << 32) | rl[1]);
+}
+__device__ __forceinline__ void swap16(double& a, double& b) {
+    const long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+    const auto rl = __builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)y, false, false);
+    const auto rh = __builtin_amdgcn_permlane16_swap((unsigned)(x >> 32), (unsigned)(y >> 32), false, false);
+    a = __longlong_as_double(((long long)rh[0] << 32) | rl[0]);
+    b = __longlong_as_double(((long long)rh[1] << 32) | rl[1]);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sums 16 values over the wave's lanes; lane l returns the total of value
+// (l >> 2) & 15. Each level pairs every lane with one in the other half of its
+// group (32, 16, 8 and 4 lanes; the partners' sets of summed lanes are
+// disjoint), keeps half of its values and adds the partner's copy of them;
+// the last two levels sum the quad.
+__device__ __forceinline__ double wave_reduce16(const double (&d)[16], int lane) {
+    const bool b3 = lane & 8, b2 = lane & 4;
+    double e[8], f[4], g[2];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {  // lanes < 32 keep values 0-7, the others 8-15
+        double a = d[i], b = d[i + 8];
+        swap32(a, b);
+        e[i] = a + b;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double a = e[i], b = e[i + 4];
+        swap16(a, b);
+        f[i] = a + b;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) g[i] = (b3 ? f[i + 2] : f[i]) + dpp_f64<0x140>(b3 ? f[i] : f[i + 2]);
+    double h = (b2 ? g[1] : g[0]) + dpp_f64<0x141>(b2 ? g[0] : g[1]);
+    h += dpp_f64<0x4E>(h);
+    h += dpp_f64<0xB1>(h);
+    return h;
+}
+
+// 1/sqrt(x) and 1/x for finite x > 0: the hardware estimates (v_rsq_f64,
+// v_rcp_f64) refined by two Newton steps, y += y (1/2 - x y^2 / 2) and
+// y += y (1 - x y); within one unit in the last place over 1e-35..1e35.
+__device__ __forceinline__ double rsq_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) y = fma(y, fma(-hx * y, y, 0.5), y);
+    return y;
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) y = fma(y, fma(-x, y, 1.0), y);
+    return y;
+}
+
+// Column pair k of step s: in-block steps (KIND 0, s < 7) pair the columns of
+// each block in the rr_pair order of 8 (pairs 0-3 block bi, 4-7 block bj);
+// cross steps (KIND 1, s < 8) pair column k of bi with column (k + s) & 7 of bj.
+__host__ __device__ constexpr int row_pair_a(int kind, int s, int k) {
+    return kind ? k : (k & 3) == 0 ? s + (k & 4) * 2 : (s + (k & 3)) % 7 + (k & 4) * 2;
+}
+__host__ __device__ constexpr int row_pair_b(int kind, int s, int k) {
+    return kind ? 8 + ((k + s) & 7) : (k & 3) == 0 ? 7 + (k & 4) * 2 : (s - (k & 3) + 7) % 7 + (k & 4) * 2;
+}
+
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Exact norms^2 of the 16 columns into every wave's rn (one reduction).
+__device__ __forceinline__ void row_norms(const double2 (&v)[16], RowLds& rl, int& ph, int lane, int wave) {
+    double d[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) d[q] = cnorm(v[q]);
+    const double h = wave_reduce16(d, lane);
+    const int p = ph++ & 1;
+    if ((lane & 3) == 0) rl.red[p][lane >> 2][wave] = h;
+    __syncthreads();
+    const double t = row_total(rl, p, lane);
+    if (lane < 16) rl.rn[wave][lane] = t;
+    wave_lds_fence();
+}
+
+// One step of 8 pair rotations (KIND, S as in row_pair_a). cvalid: bit q set
+// when column q exists. Accumulates visited / rotated pairs and the rotation
+// flags (bit 0 any, bit 1 a coupling above big_tol).
+template <int KIND, int S>
+__device__ __forceinline__ void row_step(double2 (&v)[16], RowLds& rl, int& ph, int lane, int wave, unsigned cvalid, int m,
+                                double floor2, double big_tol, int& flags, int& nvis, int& nrot,
+                                unsigned long long* tm) {
+    // tm (thread 0 of the profiled workgroup): cycles of the step's parts
+    unsigned long long tc = tm ? __builtin_readcyclecounter() : 0;
+    auto mark = [&](int q) {
+        if (tm) {
+            const unsigned long long t = __builtin_readcyclecounter();
+            tm[q] += t - tc;
+            tc = t;
+        }
+    };
+    double d[16];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const double2 x = v[row_pair_a(KIND, S, k)], y = v[row_pair_b(KIND, S, k)];
+        d[2 * k] = x.x * y.x + x.y * y.y;  // conj(x) y
+        d[2 * k + 1] = x.x * y.y - x.y * y.x;
+    }
+    const double h = wave_reduce16(d, lane);
+    const int p = ph++ & 1;
+    if ((lane & 3) == 0) rl.red[p][lane >> 2][wave] = h;
+    mark(0);
+    __syncthreads();
+    mark(1);
+    // pair k = (lane >> 1) & 7: the even lane of each lane pair holds Re x^H y
+    // and takes Im from its odd neighbour (every 16 lanes compute the same 8
+    // rotations; only the even lanes' results are used)
+    const int k = (lane >> 1) & 7, a = row_pair_a(KIND, S, k), b = row_pair_b(KIND, S, k);
+    const double gr = row_total(rl, p, lane), gi = dpp_f64<0xB1>(gr);
+    const bool vis = ((cvalid >> a) & 1) && ((cvalid >> b) & 1);
+    double al = rl.rn[wave][a], be = rl.rn[wave][b];
+    double cs = 1.0, sn = 0.0, phx = 1.0, phy = 0.0;
+    bool rot = false, big = false, need = false;
+    if (vis && al > floor2 && be > floor2) {
+        // the rotation of pair_rotate_carried with squared tests and refined
+        // hardware reciprocals in place of divides and square roots: the phase
+        // and cs keep unit modulus and cs^2 (1 + t^2) = 1 to rounding, so the
+        // applied transformation stays unitary
+        const double g2 = gr * gr + gi * gi, ab = al * be, tol = jacobi_tol(m);
+        if (g2 > 0.0 && g2 > tol * tol * ab) {
+            rot = true;
+            big = g2 > big_tol * big_tol * ab;
+            const double rg = rsq_nr(g2), gabs = g2 * rg;
+            phx = gr * rg;  // conj(gamma)/|gamma|
+            phy = -gi * rg;
+            const double zeta = 0.5 * (be - al) * rg;
+            const double w = 1.0 + zeta * zeta;
+            const double t = (zeta >= 0 ? 1.0 : -1.0) * rcp_nr(fabs(zeta) + w * rsq_nr(w));
+            cs = rsq_nr(1.0 + t * t);
+            sn = cs * t;
+            const double al1 = al - t * gabs, be1 = be + t * gabs;
+            need = al1 < al * (1.0 / 64) || be1 < be * (1.0 / 64);
+            al = al1;
+            be = be1;
+        }
+    }
+    mark(2);
+    // unrotated pairs publish the identity (cs 1, sn 0, phase 1), which the
+    // application below reproduces exactly
+    const bool writer = lane < 16 && !(lane & 1);
+    if (writer) {
+        rl.par[wave][k][0] = cs;
+        rl.par[wave][k][1] = sn;
+        rl.par[wave][k][2] = phx;
+        rl.par[wave][k][3] = phy;
+        if (rot) {
+            rl.rn[wave][a] = al;
+            rl.rn[wave][b] = be;
+        }
+    }
+    const unsigned long long even = 0x5555ull;  // the writers' lanes
+    const unsigned long long rotm = __ballot(rot) & even, bigm = __ballot(big) & even;
+    const unsigned long long needm = __ballot(need) & even, vism = __ballot(vis) & even;
+    wave_lds_fence();
+    mark(3);
+    if (rotm) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const double c0 = rl.par[wave][q][0], s0 = rl.par[wave][q][1];
+            const double2 e = make_double2(rl.par[wave][q][2], rl.par[wave][q][3]);
+            const double2 x = v[row_pair_a(KIND, S, q)], y = cmul(v[row_pair_b(KIND, S, q)], e);
+            v[row_pair_a(KIND, S, q)] = make_double2(c0 * x.x - s0 * y.x, c0 * x.y - s0 * y.y);
+            v[row_pair_b(KIND, S, q)] = make_double2(s0 * x.x + c0 * y.x, s0 * x.y + c0 * y.y);
+        }
+    }
+    mark(4);
+    nvis += __popcll(vism);
+    nrot += __popcll(rotm);
+    flags |= (rotm ? 1 : 0) | (bigm ? 2 : 0);
+    if (needm) row_norms(v, rl, ph, lane, wave);  // identical in every wave: a uniform branch
+}
+
+template <int KIND, int S, int N>
+__device__ __forceinline__ void row_steps(double2 (&v)[16], RowLds& rl, int& ph, int lane, int wave, unsigned cvalid, int m,
+                                 double floor2, double big_tol, int& flags, int& nvis, int& nrot,
+                                 unsigned long long* tm) {
+    if constexpr (S < N) {
+        row_step<KIND, S>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
+        row_steps<KIND, S + 1, N>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
+    }
+}
+
+// A whole tournament round of block pair (bi, bj) in the row layout: load the
+// 16 columns into the caller's registers v (block bi only when load_a; else v
+// still holds it from the previous round), the in-block pairs (once per sweep),
+// the cross pairs, write back (block bi only when store_a). Returns the
+// rotation flags; counts visited and rotated pairs. `lap` marks the profile phases (12 load, 13 in-block, 14 cross,
+// 15 write-back). ph counts the LDS reductions of the workgroup: consecutive
+// reductions alternate between the two partial buffers, so one barrier each
+// suffices (a wave can be at most one reduction ahead of another).
+template <bool XL, class Lap>
+__device__ __forceinline__ int round_rows(double2* __restrict__ Wg, int ld, int m, int c, int bi, int bj, bool inblock,
+                                 double floor2, double big_tol, RowLds& rl, int& ph, int& nvis, int& nrot,
+                                 unsigned long long* tm, double2 (&v)[16], bool load_a, bool store_a, Lap&& lap) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    unsigned cvalid = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
+        const bool ok = col < c;
+        cvalid |= ok ? 1u << q : 0u;
+        if (q >= kBlk || load_a)
+            v[q] = (ok && tid < m) ? gld<XL>(Wg + (std::size_t)col * ld + tid) : make_double2(0.0, 0.0);
+    }
+    int flags = 0;
+    row_norms(v, rl, ph, lane, wave);
+    lap(12);
+    if (inblock)
+        row_steps<0, 0, kBlk - 1>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
+    lap(13);
+    row_steps<1, 0, kBlk>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
+    lap(14);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
+        if ((q >= kBlk || store_a) && ((cvalid >> q) & 1) && tid < m) Wg[(std::size_t)col * ld + tid] = v[q];
+    }
+    lap(15);
+    return flags;
 }
 
 // How a block-Jacobi SVD is synchronised with others in the same launch. Alone:
@@ -688,6 +973,13 @@ struct SvdMode {
     int* err = nullptr;         // raised by a round wait that gave up
     long long spin_limit = 0;   // polls before a round wait gives up
     int naps = 1;               // 64-cycle sleeps between polls
+    // > 0: the sweeps end after a sweep whose rotations all had a relative coupling
+    // |x^H y| / (|x| |y|) of at most stop_ratio (one-sided Jacobi converges
+    // quadratically, so the next sweep's couplings are O(stop_ratio^2) and below
+    // the rotation threshold); 0: they end after a sweep that rotates nothing
+    double stop_ratio = 0.0;
+    bool rows = true;           // rounds in the row layout (round_rows); false: columns through LDS
+    bool step_prof = false;     // with prof: cycle profile of the row-layout steps (slows them)
 };
 
 // Worker holding block b in tournament round r of rr_pair with one pair per
@@ -698,6 +990,36 @@ __device__ inline int rr_holder(int r, int b, int cp) {
     if (b == L || b == r) return 0;
     const int k1 = ((b - r) % L + L) % L;
     return (k1 >= 1 && k1 <= cp / 2 - 1) ? k1 : ((r - b) % L + L) % L;
+}
+
+// Resident-block tournament of 2P blocks on P workers (P a power of two, one
+// pair per worker and round; 2P - 1 rounds, every block pair once). Level l
+// (s = P >> l rounds) splits the blocks into sets of 2s, served by s workers
+// each: worker j of a set keeps block j of the set for the whole level and in
+// round t meets block s + (j + t) mod s, which it receives from worker j + 1
+// of the set. So inside a level each worker loads, stores and waits for one
+// block per round; the other stays in its registers.
+__device__ inline void tour_level(int r, int P, int& s, int& t) {
+    int base = 0;
+    s = P;
+    while (s > 1 && r >= base + s) {
+        base += s;
+        s >>= 1;
+    }
+    t = r - base;
+}
+__device__ inline void tour_pair(int r, int w, int P, int& a, int& b) {
+    int s, t;
+    tour_level(r, P, s, t);
+    const int g = w / s, j = w % s;
+    a = g * 2 * s + j;
+    b = g * 2 * s + s + (j + t) % s;
+}
+__device__ inline int tour_holder(int r, int blk, int P) {
+    int s, t;
+    tour_level(r, P, s, t);
+    const int g = blk / (2 * s), off = blk % (2 * s);
+    return off < s ? g * s + off : g * s + (off - s - t + s) % s;
 }
 
 // Thread 0 of a workgroup waits until a round counter reaches `target`; every
@@ -780,18 +1102,31 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     // only by its current holder and read only by the next one. The last round of
     // a sweep keeps the full barrier (the rotation flags are read after it).
     const bool p2p = mode.epochs != nullptr && nworkers == P && mode.active;
+    // row layout, at most one pair per worker and P a power of two: the
+    // resident-block tournament (tour_pair), else rr_pair. The ordering depends
+    // only on the shape, so every synchronisation mode applies the same rotations.
+    const bool tour = mode.rows && nworkers >= P && (P & (P - 1)) == 0;
+    double2 rv[16];  // the row layout's registers, kept across rounds for a resident block
     int ep = p2p ? __hip_atomic_load(&mode.epochs[(std::size_t)me * 64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-    int sweeps_done = 0;
+    int sweeps_done = 0, ph = 0;
     unsigned long long nvis = 0, nrot = 0;
     for (int sweep = 0; sweep < 40; ++sweep) {
         if (working) ++sweeps_done;
+        if (prof && working) sh.tm[kProfHist + 2 * kHistSweeps + min(sweep, kHistSweeps - 1)] += 1;
         for (int r = 0; r < rounds; ++r) {
             if (p2p && r > 0) {
                 const unsigned long long tw = prof ? wall_clock64() : 0;
                 if (tid == 0) {
-                    int bi, bj;
-                    rr_pair(r, me, nbp, bi, bj);
-                    const int pa = rr_holder(r - 1, bi, nbp), pb = rr_holder(r - 1, bj, nbp);
+                    int bi, bj, pa, pb;
+                    if (tour) {
+                        tour_pair(r, me, P, bi, bj);
+                        pa = tour_holder(r - 1, bi, P);
+                        pb = tour_holder(r - 1, bj, P);
+                    } else {
+                        rr_pair(r, me, nbp, bi, bj);
+                        pa = rr_holder(r - 1, bi, nbp);
+                        pb = rr_holder(r - 1, bj, nbp);
+                    }
                     if (pa != me)
                         wait_counter<XL>(&mode.epochs[(std::size_t)pa * 64], ep, mode.err, mode.spin_limit, mode.naps);
                     if (pb != me)
@@ -808,12 +1143,22 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
             if (working && r < nbp - 1)
             for (int pi = me; pi < P; pi += nworkers) {
                 int bi, bj;
-                rr_pair(r, pi, nbp, bi, bj);
+                bool load_a = true, store_a = true;
+                if (tour) {
+                    tour_pair(r, pi, P, bi, bj);
+                    int pa, pb;
+                    if (r > 0) {  // bi stayed in the registers since the last round
+                        tour_pair(r - 1, pi, P, pa, pb);
+                        load_a = pa != bi;
+                    }
+                    if (r + 1 < rounds && r + 1 < nbp - 1) {  // bi stays for the next round
+                        tour_pair(r + 1, pi, P, pa, pb);
+                        store_a = pa != bi;
+                    }
+                } else {
+                    rr_pair(r, pi, nbp, bi, bj);
+                }
                 if (tid == 0) sh.rot = 0;
-                double2 xa[kRowsPerLane], ya[kRowsPerLane];
-                // LDS column q <-> block (q < 8 ? bi : bj), member q & 7; a column's
-                // rows go through registers with all of a lane's loads in flight at
-                // once (a load-store loop would wait out each load in turn)
                 unsigned long long tr0 = prof ? wall_clock64() : 0;
                 auto lap = [&](int slot) {  // round phases of workgroup 0
                     if (prof) {
@@ -822,6 +1167,23 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                         tr0 = t;
                     }
                 };
+                int rflags = 0, wrot = 0;
+                if (mode.rows) {
+                    int nv = 0, nr = 0;
+                    rflags = round_rows<XL>(Wg, ld, m, c, bi, bj, r == 0, floor2, mode.stop_ratio,
+                                            *reinterpret_cast<RowLds*>(lds), ph, nv, nr,
+                                            (prof && mode.step_prof) ? sh.tm + kProfStep : nullptr, rv, load_a,
+                                            store_a, lap);
+                    if (tid == 0) {  // every wave holds the same counts
+                        nvis += (unsigned long long)nv;
+                        nrot += (unsigned long long)nr;
+                        wrot = nr;
+                    }
+                } else {
+                double2 xa[kRowsPerLane], ya[kRowsPerLane];
+                // LDS column q <-> block (q < 8 ? bi : bj), member q & 7; a column's
+                // rows go through registers with all of a lane's loads in flight at
+                // once (a load-store loop would wait out each load in turn)
                 for (int q = wave; q < 2 * kBlk; q += kDistThreads / 64) {
                     const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
                     if (col < c) {
@@ -831,7 +1193,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 }
                 __syncthreads();
                 lap(12);
-                bool rotated = false;
+                bool rotated = false, big = false;
                 if (r == 0) {  // pairs inside each block, once per sweep
                     for (int s = 0; s < kBlk - 1; ++s) {
                         const int half = wave >> 2, k = wave & 3;  // waves 0-3 block bi, 4-7 block bj
@@ -843,9 +1205,10 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                             double2* py = lds + (std::size_t)(half * kBlk + j) * m;
                             load_col(xa, px, m, lane);
                             load_col(ya, py, m, lane);
-                            if (pair_rotate(xa, ya, m, floor2)) {
+                            if (pair_rotate(xa, ya, m, floor2, mode.stop_ratio, &big)) {
                                 rotated = true;
                                 ++nrot;
+                                ++wrot;
                                 store_col(xa, px, m, lane);
                                 store_col(ya, py, m, lane);
                             }
@@ -874,9 +1237,10 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                         double2* py = lds + (std::size_t)(kBlk + j) * m;
                         load_col(ya, py, m, lane);
                         double be = sh.bn[j];
-                        if (pair_rotate_carried(xa, ya, m, al, be, floor2)) {
+                        if (pair_rotate_carried(xa, ya, m, al, be, floor2, mode.stop_ratio, big)) {
                             rotated = true;
                             ++nrot;
+                            ++wrot;
                             store_col(ya, py, m, lane);
                             if (lane == 0) sh.bn[j] = be;
                         }
@@ -885,13 +1249,9 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                     __syncthreads();
                 }
                 if (mine) store_col(xa, lds + (std::size_t)wave * m, m, lane);
-                if (rotated && lane == 0) sh.rot = 1;
+                if (rotated && lane == 0) atomicOr(&sh.rot, big ? 3 : 1);
                 __syncthreads();
                 lap(14);
-                if (tid == 0 && sh.rot) {
-                    atomicOr(&flags[sweep & 1], 1);
-                    if (mode.gflags) atomicOr(&mode.gflags[sweep & 1], 1);
-                }
                 for (int q = wave; q < 2 * kBlk; q += kDistThreads / 64) {
                     const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
                     if (col < c) {
@@ -901,6 +1261,16 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 }
                 __syncthreads();  // the next pair of this worker reuses the LDS columns
                 lap(15);
+                rflags = sh.rot;
+                }
+                if (mode.prof && lane == 0 && wrot)
+                    atomicAdd(&sh.tm[kProfHist + min(sweep, kHistSweeps - 1)], (unsigned long long)wrot);
+                if (tid == 0 && rflags) {
+                    atomicOr(&flags[sweep & 1], rflags);
+                    if (mode.gflags) atomicOr(&mode.gflags[sweep & 1], rflags);
+                    if (mode.prof && (rflags & 2))
+                        sh.tm[kProfHist + kHistSweeps + min(sweep, kHistSweeps - 1)] += 1;
+                }
             }
             if (p2p && r < rounds - 1) {
                 const unsigned long long tpub = prof ? wall_clock64() : 0;
@@ -942,8 +1312,9 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
         sync();
         if (me == 0 && tid == 0) flags[sweep & 1] = 0;
         if (mode.grid_leader && tid == 0 && mode.gflags) mode.gflags[sweep & 1] = 0;
-        if (!f) working = false;
-        if (!gf) break;
+        // bit 1: a rotation above stop_ratio (every rotation when stop_ratio = 0)
+        if (!(f & 2)) working = false;
+        if (!(gf & 2)) break;
     }
     if (prof) {
         sh.tm[2] += wall_clock64() - tp;
@@ -1645,6 +2016,9 @@ struct ExecArgs {
     int* err;           // raised by a GroupSync wait that gave up
     long long spin_limit;  // GroupSync polls before giving up
     int naps;              // 64-cycle sleeps between the polls of a wait
+    double stop_ratio;     // SvdMode::stop_ratio of every SVD
+    int rows;              // SvdMode::rows of every SVD
+    int step_prof;         // SvdMode::step_prof of the profiled workgroup
     int* epochs;        // [G][W][64] round counters of the point-to-point rounds (per-group layout)
     int dataflow;       // per-group layout without snapshots: no super-step barriers; an update
                         // waits for the done flags of its predecessors instead
@@ -1814,6 +2188,8 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                                                sh);
                 } else {
                     SvdMode mode;  // own tournament length, own convergence
+                    mode.stop_ratio = a.stop_ratio;
+                    mode.rows = a.rows != 0;
                     svd_blocks(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, 0, 1, bs, lds,
                                sh, mode, (a.aux ? a.aux + grp * mat_sz : nullptr), a.taus + (std::size_t)grp * a.ld);
                 }
@@ -1843,6 +2219,9 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 mark(0);
                 SvdMode mode;
                 mode.prof = blockIdx.x == 0;
+                mode.stop_ratio = a.stop_ratio;
+                mode.rows = a.rows != 0;
+                mode.step_prof = a.step_prof != 0;
                 mode.epochs = a.epochs ? a.epochs + (std::size_t)grp * nwg * 64 : nullptr;
                 mode.err = a.err;
                 mode.spin_limit = a.spin_limit;
@@ -1875,6 +2254,9 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
             mark(0);
             // 2-3. lockstep SVD; svd_blocks opens with a grid barrier
             SvdMode mode;
+            mode.stop_ratio = a.stop_ratio;
+            mode.rows = a.rows != 0;
+            mode.step_prof = a.step_prof != 0;
             mode.rounds = a.rounds[st];
             mode.gflags = a.gflags;
             mode.grid_leader = blockIdx.x == 0;
@@ -2004,9 +2386,23 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     // same results; no faster on the n = 128 chain, slower on mixed-chi batches)
     const char* df = std::getenv("TESSERA_EXEC_DATAFLOW");
     ex_dataflow_ = df && df[0] == '1';
-    // test knob: polls before a group barrier wait gives up (forces the error path)
-    const char* nap = std::getenv("TESSERA_EXEC_POLL_NAPS");  // knob: 64-cycle sleeps between polls
+    // knob: 64-cycle sleeps between the polls of a group barrier or round wait
+    const char* nap = std::getenv("TESSERA_EXEC_POLL_NAPS");
     ex_naps_ = nap ? std::max(1, std::atoi(nap)) : 1;
+    // knob: the Jacobi sweeps of the executor's SVDs end after a sweep whose
+    // rotations all had a relative coupling of at most this value (0: after a
+    // sweep that rotates nothing)
+    const char* st = std::getenv("TESSERA_SVD_STOP");
+    svd_stop_ = st ? std::max(0.0, std::atof(st)) : 0.0;
+    // knob: 0 runs the executor's tournament rounds with the columns in LDS
+    // instead of the row layout
+    const char* rw = std::getenv("TESSERA_SVD_ROWS");
+    svd_rows_ = !(rw && rw[0] == '0');
+    // knob: 1 adds the cycle profile of the row-layout rotation steps to the
+    // executor profile (its timer reads slow the profiled workgroup)
+    const char* sp = std::getenv("TESSERA_EXEC_STEP_PROFILE");
+    ex_step_prof_ = sp && sp[0] == '1';
+    // test knob: polls before a group barrier wait gives up (forces the error path)
     const char* sl = std::getenv("TESSERA_EXEC_SPIN_LIMIT");
     ex_spin_limit_ = sl ? std::max(0ll, std::atoll(sl)) : kSpinLimit;
     // knob, off by default: 1 synchronises each group of the grouped executor
@@ -2182,7 +2578,8 @@ void Tebd::two_site(int p, const Gate4& g, const double* wts, int edge) {
         const int nb = (ccap + kBlk - 1) / kBlk;
         const int nbp = std::max(2, nb + (nb & 1));
         const unsigned nwg = (unsigned)(nbp / 2) * (unsigned)xcd_stride_;
-        const std::size_t shm = (std::size_t)2 * kBlk * mcap * sizeof(double2);
+        // the LDS columns of a round, or the row layout's reduction area
+        const std::size_t shm = std::max((std::size_t)2 * kBlk * mcap * sizeof(double2), sizeof(RowLds));
         int* flags = flags_.p;
         int pp = p, cc = chi_;
         unsigned long long* stats = stats_.p;
@@ -2409,7 +2806,7 @@ static void exec_shape(int maxcap, int req, bool solo, bool group_sync, int work
     const std::size_t svd_bytes = (std::size_t)2 * kBlk * (2 * maxcap) * sizeof(double2);
     // solo: room for a kSmall x kSmall theta of svd_small
     const std::size_t small_bytes = solo ? (std::size_t)kSmall * kSmall * sizeof(double2) : 0;
-    shm = std::max(svd_bytes, small_bytes);
+    shm = std::max({svd_bytes, small_bytes, sizeof(RowLds)});
     int per_cu = 0, cus = 0, dev = 0;
     TQ_HIP(hipGetDevice(&dev));
     TQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -2719,6 +3116,9 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.err = t0.ex_bar_.p;  // the error word, then [G][2] barrier words
     x.spin_limit = t0.ex_spin_limit_;
     x.naps = t0.ex_naps_;
+    x.stop_ratio = t0.svd_stop_;
+    x.rows = t0.svd_rows_ ? 1 : 0;
+    x.step_prof = t0.ex_step_prof_ ? 1 : 0;
     x.dataflow = (group_sync && t0.ex_dataflow_ && snap_list.empty()) ? 1 : 0;
     x.done = nullptr;
     if (x.dataflow) {
@@ -2898,7 +3298,7 @@ std::vector<double> Tebd::exec_profile() {
     TQ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     const double ms_per_tick = khz > 0 ? 1.0 / khz : 0.0;
     for (int q = 0; q < kProfSlots; ++q)
-        out[q] = (q == 7 || q == 9 || q == 11) ? (double)h[q] : (double)h[q] * ms_per_tick;
+        out[q] = (q == 7 || q == 9 || q == 11 || q >= kProfHist) ? (double)h[q] : (double)h[q] * ms_per_tick;
     return out;
 }
 

@@ -195,6 +195,9 @@ private:
     bool ex_armed_ = false;                // a launch whose barrier error word is still unread
     long long ex_spin_limit_ = 1ll << 24;  // GroupSync polls before a wait gives up
     int ex_naps_ = 1;                      // 64-cycle sleeps between the polls of a wait
+    double svd_stop_ = 0.0;                // SvdMode::stop_ratio of the executor's SVDs
+    bool svd_rows_ = true;                 // SvdMode::rows of the executor's SVDs
+    bool ex_step_prof_ = false;            // cycle profile of the row-layout steps
     bool ex_p2p_ = true;                   // per-group layout: point-to-point tournament rounds
     DevBuf<int> ex_ep_;                    // their round counters
     bool ex_dataflow_ = false;             // per-group layout: updates wait for their predecessors only

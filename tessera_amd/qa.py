@@ -472,6 +472,23 @@ class TebdSession:
         _check(lib().tessera_tebd_exec_profile(self._h, _p(out), out.size))
         return dict(zip(self.PROFILE_SLOTS, out.tolist()))
 
+    HIST_SWEEPS = 24
+
+    def sweep_profile(self) -> dict:
+        """Jacobi convergence of the executor's first workgroup since creation, per
+        sweep index (the last index also counts every later sweep): `rotations`
+        its waves performed, `big_rounds` its rounds with a rotation above the
+        stop ratio (every rotating round when the ratio is 0), `svds` its SVDs
+        that ran the sweep; `step_cycles`: its thread 0's shader cycles in the
+        parts of the row-layout steps."""
+        n = len(self.PROFILE_SLOTS)
+        out = np.zeros(n + 3 * self.HIST_SWEEPS + 5)
+        _check(lib().tessera_tebd_exec_profile(self._h, _p(out), out.size))
+        h = out[n:n + 3 * self.HIST_SWEEPS].reshape(3, self.HIST_SWEEPS)
+        steps = dict(zip(("dots_reduce", "barrier", "rotation", "broadcast", "apply"),
+                         out[n + 3 * self.HIST_SWEEPS:].tolist()))
+        return {"rotations": h[0], "big_rounds": h[1], "svds": h[2], "step_cycles": steps}
+
     @property
     def updates_per_step(self) -> int:
         return int(lib().tessera_tebd_updates_per_step(self._h))

@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--epochs", type=int, default=60, help="learned workload: training epochs")
     ap.add_argument("--fractions", default="0.25,0.5", help="learned workload: shortened anneal lengths")
     ap.add_argument("--strengths", default="0.5,1.0", help="learned workload: warm-start bias strengths")
+    ap.add_argument("--no-lq", action="store_true",
+                    help="tebd: Jacobi directly on theta, without the LQ preconditioning of the block SVD")
     ap.add_argument("--step-profile", action="store_true",
                     help="tebd: add the executor's cycle profile of the Jacobi rotation steps (slows workgroup 0)")
     ap.add_argument("--exec-workers", type=int, default=0,
@@ -566,7 +568,9 @@ def sweep_summary(h0, h1, rounds):
     last = int(np.flatnonzero(svds)[-1]) + 1 if svds.any() else 0
     cyc = {k: round((h1["step_cycles"][k] - h0["step_cycles"][k]) / max(rounds, 1), 1) for k in h1["step_cycles"]}
     out = {"svds": [int(x) for x in svds[:last]],
-           "rotations_per_svd": [round(float(r / v), 1) if v else 0.0 for r, v in zip(rot[:last], svds[:last])]}
+           "rotations_per_svd": [round(float(r / v), 1) if v else 0.0 for r, v in zip(rot[:last], svds[:last])],
+           "rounds_ms": round(h1["rounds_ms"] - h0["rounds_ms"], 3),
+           "sweep_end_ms": round(h1["sweep_end_ms"] - h0["sweep_end_ms"], 3)}
     if any(cyc.values()):
         out["step_cycles_per_round"] = cyc
     return out
@@ -575,6 +579,8 @@ def sweep_summary(h0, h1, rounds):
 def tebd_timed(args, model, sched, k0, sess):
     """The headline measurement on an open session (closed by the caller)."""
     sess.set_executor(args.executor, args.exec_workers)
+    if args.no_lq:
+        sess.set_svd_lq(False)
     sess.randomize(SEED, 40.0)
     ups = sess.updates_per_step
     a = sched.a[k0:k0 + args.warmup + args.steps]
@@ -642,7 +648,8 @@ def tebd_timed(args, model, sched, k0, sess):
                                             and os.environ.get("TESSERA_EXEC_P2P", "1") != "0"),
                   "workgroup0_phases": {k: round(pr1[k] - pr0[k], 3) for k in pr1},
                   "workgroup0_sweeps": sweep_summary(sw0, sess.sweep_profile(), pr1["rounds"] - pr0["rounds"]),
-                  "svd_stop_ratio": float(os.environ.get("TESSERA_SVD_STOP", "0") or 0)}
+                  "svd_stop_ratio": float(os.environ.get("TESSERA_SVD_STOP", "0") or 0),
+                  "svd_lq": not args.no_lq}
     else:
         kname = "k_svd_dist"
         kernel = "per-update launches; timed: the SVD launches (k_svd_dist / k_svd)"

@@ -235,7 +235,7 @@ int tessera_tebd_last_run_ms(tessera_tebd* t, double* out_ms);
  * updates placed in them (updates / (super-steps * groups) = slot fill). */
 int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* out_updates);
 /* Diagnostics (no upstream counterpart): phase profile of the lockstep
- * executor's first workgroup since creation, first `count` of 94 slots: ms in
+ * executor's first workgroup since creation, first `count` of 96 slots: ms in
  * 0 snapshots + theta, 1 LQ preconditioning, 2 Jacobi sweeps, 3 whole SVD,
  * 4 wait after the SVD, 5 reconstruction, 6 wait at the super-step end,
  * 8 waits at tournament-round barriers, 10 whole launches; counts in 7
@@ -247,7 +247,8 @@ int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* o
  * 65 + s SVDs that ran sweep s; shader cycles of thread 0 in the parts of the
  * row-layout rotation steps: 89 products and wave reduction, 90 barrier,
  * 91 totals and rotations, 92 rotation broadcast, 93 application (recorded
- * with TESSERA_EXEC_STEP_PROFILE=1 set when the session is created). */
+ * with TESSERA_EXEC_STEP_PROFILE=1 set when the session is created); ms in
+ * 94 whole tournament rounds, 95 the flag checks between sweeps. */
 int tessera_tebd_exec_profile(tessera_tebd* t, double* out, size_t count);
 /* Number of two-site updates one Trotter step performs for this model. */
 size_t tessera_tebd_updates_per_step(const tessera_tebd* t);

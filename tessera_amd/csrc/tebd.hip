@@ -468,7 +468,8 @@ constexpr int kStatSlots = 1024;   // per-workgroup accounting slots (largest gr
 constexpr int kHistSweeps = 24;  // sweep histogram: sweeps past the last fold into it
 constexpr int kProfHist = 17;    // first histogram slot
 constexpr int kProfStep = kProfHist + 3 * kHistSweeps;  // row-layout step parts (cycles)
-constexpr int kProfSlots = kProfStep + 5;
+constexpr int kProfRound = kProfStep + 5;  // whole rounds, sweep ends (wall ticks)
+constexpr int kProfSlots = kProfRound + 2;
 
 struct SvdShared {
     unsigned long long tm[kProfSlots];  // phase profile (workgroup 0, thread 0)
@@ -1114,6 +1115,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
         if (working) ++sweeps_done;
         if (prof && working) sh.tm[kProfHist + 2 * kHistSweeps + min(sweep, kHistSweeps - 1)] += 1;
         for (int r = 0; r < rounds; ++r) {
+            const unsigned long long tround = prof ? wall_clock64() : 0;
             if (p2p && r > 0) {
                 const unsigned long long tw = prof ? wall_clock64() : 0;
                 if (tid == 0) {
@@ -1127,13 +1129,14 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                         pa = rr_holder(r - 1, bi, nbp);
                         pb = rr_holder(r - 1, bj, nbp);
                     }
+                    // device scope: the polls are acquire loads, so the producers'
+                    // blocks (behind their release) are visible to the whole workgroup
+                    // after the barrier below; XCD-local: the blocks are read with
+                    // L1-bypassing loads
                     if (pa != me)
                         wait_counter<XL>(&mode.epochs[(std::size_t)pa * 64], ep, mode.err, mode.spin_limit, mode.naps);
                     if (pb != me)
                         wait_counter<XL>(&mode.epochs[(std::size_t)pb * 64], ep, mode.err, mode.spin_limit, mode.naps);
-                    // device scope: the producers' blocks, after their release; XCD-local:
-                    // the blocks are read with L1-bypassing loads below
-                    if constexpr (!XL) __threadfence();
                 }
                 __syncthreads();
                 if (prof) sh.tm[8] += wall_clock64() - tw;
@@ -1277,8 +1280,9 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 // publish this round's blocks: every wave waits until its write-back
                 // stores are complete in the L2 (s_waitcnt of all counters; stores
                 // count in vmcnt on CDNA), the workgroup barrier follows, and thread 0's
-                // agent-scope release then writes the L2 back before it stores the
-                // counter (its own fence orders only its own stores)
+                // agent-scope release store then writes the L2 back (the whole XCD's
+                // dirty lines, these stores among them) before it stores the counter;
+                // no further fence (an acquire here would only invalidate the L2)
                 ++ep;
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
@@ -1287,7 +1291,6 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                         __hip_atomic_store(&mode.epochs[(std::size_t)me * 64], ep, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
                     } else {
-                        __threadfence();
                         __hip_atomic_store(&mode.epochs[(std::size_t)me * 64], ep, __ATOMIC_RELEASE,
                                            __HIP_MEMORY_SCOPE_AGENT);
                     }
@@ -1304,7 +1307,9 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                     sh.tm[9] += 1;
                 }
             }
+            if (prof) sh.tm[kProfRound] += wall_clock64() - tround;
         }
+        const unsigned long long tend = prof ? wall_clock64() : 0;
         const int f = __hip_atomic_load(&flags[sweep & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int gf = mode.gflags ? __hip_atomic_load(&mode.gflags[sweep & 1], __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_AGENT)
@@ -1312,6 +1317,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
         sync();
         if (me == 0 && tid == 0) flags[sweep & 1] = 0;
         if (mode.grid_leader && tid == 0 && mode.gflags) mode.gflags[sweep & 1] = 0;
+        if (prof) sh.tm[kProfRound + 1] += wall_clock64() - tend;
         // bit 1: a rotation above stop_ratio (every rotation when stop_ratio = 0)
         if (!(f & 2)) working = false;
         if (!(gf & 2)) break;
@@ -2019,6 +2025,7 @@ struct ExecArgs {
     double stop_ratio;     // SvdMode::stop_ratio of every SVD
     int rows;              // SvdMode::rows of every SVD
     int step_prof;         // SvdMode::step_prof of the profiled workgroup
+    int prof_on;           // workgroup 0 records the phase profile
     int* epochs;        // [G][W][64] round counters of the point-to-point rounds (per-group layout)
     int dataflow;       // per-group layout without snapshots: no super-step barriers; an update
                         // waits for the done flags of its predecessors instead
@@ -2061,7 +2068,10 @@ struct GroupSync {
                     __builtin_amdgcn_s_sleep(1);
                 }
             } else {
-                __threadfence();  // thread 0 releases them (with the L2) at agent scope
+                // thread 0's acq_rel arrival releases every wave's stores (complete in
+                // the L2 after the s_waitcnt and barrier above) with the L2 at agent
+                // scope; the last arrival also acquires, the others acquire with
+                // their polls
                 const int gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const int prev = __hip_atomic_fetch_add(bar, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
                 if (prev == nwg - 1) {
@@ -2077,7 +2087,6 @@ struct GroupSync {
                         for (int q = 0; q < 2 * naps; ++q) __builtin_amdgcn_s_sleep(1);
                     }
                 }
-                __threadfence();  // the other workgroups' writes, after the release
             }
         }
         __syncthreads();
@@ -2133,7 +2142,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
     // 1 LQ preconditioning, 2 Jacobi sweeps, 3 whole SVD, 4 wait after the SVD,
     // 5 reconstruction, 6 wait at the super-step end, 7 super-steps with an
     // update, 8 waits at tournament-round barriers, 9 rounds, 10 kernel, 11 sweeps
-    const bool prof = blockIdx.x == 0 && tid == 0;
+    const bool prof = a.prof_on && blockIdx.x == 0 && tid == 0;
     if (prof)
         for (int q = 0; q < kProfSlots; ++q) sh.tm[q] = 0;
     const unsigned long long t_start = prof ? wall_clock64() : 0;
@@ -2218,7 +2227,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 theta_tiles(I.sites, I.off, I.cap, I.dims, u.site, u.g, M, W, a.ld, rank, nwg);
                 mark(0);
                 SvdMode mode;
-                mode.prof = blockIdx.x == 0;
+                mode.prof = a.prof_on && blockIdx.x == 0;
                 mode.stop_ratio = a.stop_ratio;
                 mode.rows = a.rows != 0;
                 mode.step_prof = a.step_prof != 0;
@@ -2262,7 +2271,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
             mode.grid_leader = blockIdx.x == 0;
             mode.active = active;
             mode.lq_panels = (a.ld + 15) / 16;
-            mode.prof = blockIdx.x == 0;
+            mode.prof = a.prof_on && blockIdx.x == 0;
             svd_blocks(W, a.ld, I.dims, active ? u.site : 0, I.chi, sv, perm, I.trunc, sflags, I.stats, rank, nwg, sync,
                        lds, sh, mode, (a.aux ? a.aux + grp * mat_sz : nullptr), a.taus + (std::size_t)grp * a.ld);
             mark(3);
@@ -2402,6 +2411,10 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     // executor profile (its timer reads slow the profiled workgroup)
     const char* sp = std::getenv("TESSERA_EXEC_STEP_PROFILE");
     ex_step_prof_ = sp && sp[0] == '1';
+    // knob: 0 switches workgroup 0's phase profile off (its timers cost that
+    // workgroup a little, and every round waits for its slowest worker)
+    const char* po = std::getenv("TESSERA_EXEC_PROFILE");
+    ex_prof_on_ = !(po && po[0] == '0');
     // test knob: polls before a group barrier wait gives up (forces the error path)
     const char* sl = std::getenv("TESSERA_EXEC_SPIN_LIMIT");
     ex_spin_limit_ = sl ? std::max(0ll, std::atoll(sl)) : kSpinLimit;
@@ -3119,6 +3132,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.stop_ratio = t0.svd_stop_;
     x.rows = t0.svd_rows_ ? 1 : 0;
     x.step_prof = t0.ex_step_prof_ ? 1 : 0;
+    x.prof_on = t0.ex_prof_on_ ? 1 : 0;
     x.dataflow = (group_sync && t0.ex_dataflow_ && snap_list.empty()) ? 1 : 0;
     x.done = nullptr;
     if (x.dataflow) {
@@ -3298,7 +3312,8 @@ std::vector<double> Tebd::exec_profile() {
     TQ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     const double ms_per_tick = khz > 0 ? 1.0 / khz : 0.0;
     for (int q = 0; q < kProfSlots; ++q)
-        out[q] = (q == 7 || q == 9 || q == 11 || q >= kProfHist) ? (double)h[q] : (double)h[q] * ms_per_tick;
+        out[q] = (q == 7 || q == 9 || q == 11 || (q >= kProfHist && q < kProfRound)) ? (double)h[q]
+                                                                                  : (double)h[q] * ms_per_tick;
     return out;
 }
 

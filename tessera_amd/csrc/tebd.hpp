@@ -198,6 +198,7 @@ private:
     double svd_stop_ = 0.0;                // SvdMode::stop_ratio of the executor's SVDs
     bool svd_rows_ = true;                 // SvdMode::rows of the executor's SVDs
     bool ex_step_prof_ = false;            // cycle profile of the row-layout steps
+    bool ex_prof_on_ = true;               // workgroup 0's phase profile
     bool ex_p2p_ = true;                   // per-group layout: point-to-point tournament rounds
     DevBuf<int> ex_ep_;                    // their round counters
     bool ex_dataflow_ = false;             // per-group layout: updates wait for their predecessors only

@@ -480,14 +480,16 @@ class TebdSession:
         its waves performed, `big_rounds` its rounds with a rotation above the
         stop ratio (every rotating round when the ratio is 0), `svds` its SVDs
         that ran the sweep; `step_cycles`: its thread 0's shader cycles in the
-        parts of the row-layout steps."""
+        parts of the row-layout steps; `rounds_ms`: its time in whole tournament
+        rounds, `sweep_end_ms` in the flag checks between sweeps."""
         n = len(self.PROFILE_SLOTS)
-        out = np.zeros(n + 3 * self.HIST_SWEEPS + 5)
+        k = n + 3 * self.HIST_SWEEPS
+        out = np.zeros(k + 5 + 2)
         _check(lib().tessera_tebd_exec_profile(self._h, _p(out), out.size))
-        h = out[n:n + 3 * self.HIST_SWEEPS].reshape(3, self.HIST_SWEEPS)
-        steps = dict(zip(("dots_reduce", "barrier", "rotation", "broadcast", "apply"),
-                         out[n + 3 * self.HIST_SWEEPS:].tolist()))
-        return {"rotations": h[0], "big_rounds": h[1], "svds": h[2], "step_cycles": steps}
+        h = out[n:k].reshape(3, self.HIST_SWEEPS)
+        steps = dict(zip(("dots_reduce", "barrier", "rotation", "broadcast", "apply"), out[k:k + 5].tolist()))
+        return {"rotations": h[0], "big_rounds": h[1], "svds": h[2], "step_cycles": steps,
+                "rounds_ms": float(out[k + 5]), "sweep_end_ms": float(out[k + 6])}
 
     @property
     def updates_per_step(self) -> int:

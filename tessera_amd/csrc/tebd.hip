@@ -2093,11 +2093,12 @@ struct GroupSync {
     }
 };
 
-// Barrier of one workgroup for code written against a barrier functor: the
-// fence makes global-memory writes of every wave visible to the others.
+// Barrier of one workgroup for code written against a barrier functor.
 struct BlockSync {
     __device__ void operator()() const {
-        __threadfence();
+        // __syncthreads carries workgroup-scope release and acquire fences, which
+        // order global memory among the workgroup's waves (they share the CU's
+        // L1); other workgroups read this data only after a grid barrier
         __syncthreads();
     }
 };
@@ -2216,7 +2217,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                     if (tid == 0) {
                         for (int q = 0; q < 2; ++q)
                             if (u.pred[q] >= 0) wait_counter(&a.done[u.pred[q]], 1, a.err, a.spin_limit);
-                        __threadfence();
+                        // the polls are acquire loads: the predecessors' sites are visible
                     }
                     __syncthreads();
                     mark(6);
@@ -2247,7 +2248,6 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 if (a.dataflow) {
                     gs();  // every workgroup's part of the reconstruction is released
                     if (rank == 0 && tid == 0) {
-                        __threadfence();
                         __hip_atomic_store(&a.done[a.up_begin[st] + grp], 1, __ATOMIC_RELEASE,
                                            __HIP_MEMORY_SCOPE_AGENT);
                     }

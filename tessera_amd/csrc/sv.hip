@@ -219,6 +219,45 @@ __device__ inline Best better(Best a, Best b) {
     return a;
 }
 
+// Amplitudes held in registers: a plain two-double struct (arrays of HIP's
+// union-based double2 are kept from register promotion in some loop forms).
+struct alignas(16) c2 {
+    double x, y;
+};
+__device__ inline c2 ldc(const double2* p) {
+    const double2 q = *p;
+    return c2{q.x, q.y};
+}
+__device__ inline void stc(double2* p, c2 v) { *p = make_double2(v.x, v.y); }
+__device__ inline c2 cm(c2 a, c2 b) { return c2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline c2 cj(c2 a) { return c2{a.x, -a.y}; }
+__device__ inline c2 c2of(double2 a) { return c2{a.x, a.y}; }
+
+// The value of lane t ^ 1 (quad permute [1, 0, 3, 2]; every lane of the wave must
+// execute it).
+__device__ inline double quad_xor1(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xB1, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Global index of the tile's local amplitude j: the low c bits are contiguous,
+// the rest sit at bit lo upward (lo < 0: a contiguous 4096 block). The map
+// scatters bits independently, so the index of j = a | b (disjoint bits) is
+// base + scat(a) + scat(b).
+__device__ inline unsigned long long tile_base(long long tile, int c, int lo) {
+    if (lo < 0) return (unsigned long long)tile << kTileBits;
+    const int lo_span = lo - c;  // tile-id bits placed in [c, lo)
+    const unsigned long long tl = (unsigned long long)tile & ((1ull << lo_span) - 1ull);
+    const unsigned long long th = (unsigned long long)tile >> lo_span;
+    return (tl << c) | (th << (lo + (kTileBits - c)));
+}
+__device__ inline unsigned long long scat(unsigned j, int c, int lo) {
+    if (lo < 0) return j;
+    return (unsigned long long)(j & ((1u << c) - 1u)) | ((unsigned long long)(j >> c) << lo);
+}
+
 // Per-tile energy tables (all 256 threads): tabE[0..127] = T1, T2; cst = C.
 __device__ void build_energy(const Term* tm, const int* off, unsigned long long base, double* tabE, double* cst) {
     const int t = threadIdx.x;
@@ -284,20 +323,92 @@ __device__ inline double2 tile_phasor(const Term* tm, const int* off, int j, con
     return f;
 }
 
-// Layout of a tile in registers: which local bits the 16 register slots hold.
-enum { kLay0 = 0, kLay1 = 1 };
-__device__ inline int lay_of(int which, int t, int r) { return which == kLay0 ? lay0(t, r) : lay1(t, r); }
+// Per-tile phase data of a pass with the register phase block, written by
+// k_sv_tile_phases just before the pass so that the pass evaluates no
+// transcendental: the base-local field f_a on each local bit a (the sum of
+// w * s_b over the couplings of bit a to base bits b; bits 0-5 are L, 6-11 H),
+// the tile constant C (base-base couplings and base fields), exp(+i s_c f_a),
+// exp(+i s_out f_a) of the L bits and exp(-i s_c C), where s_c = s_out + s_in
+// is the scale of the pass's one combined phase.
+struct alignas(16) TileRec {
+    double f[12];
+    double cst;
+    double pad;
+    c2 g[12];
+    c2 gl[6];
+    c2 cp;
+};
+static_assert(sizeof(TileRec) == 416, "tile record layout");
+constexpr int kRecChunks = (int)(sizeof(TileRec) / 16);  // 26 16-byte pieces
 
-// One full-tile exchange through LDS from layout `from` to layout `to`.
-template <int FROM, int TO>
-__device__ inline void exchange(double2 (&v)[kRegs], double2* sh) {
+__global__ void __launch_bounds__(256) k_sv_tile_phases(PassTerms pt, int c, int lo, long long ntiles, double s_c,
+                                                        double s_out, TileRec* __restrict__ rec) {
+    const long long tile = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tile >= ntiles) return;
+    const unsigned long long base = tile_base(tile, c, lo);
+    const Term* tm = pt.t;
+    TileRec r;
+#pragma unroll 1
+    for (int a = 0; a < 12; ++a) {
+        const int kb = a < 6 ? kBL : kBH, bit = a % 6;
+        double f = 0.0;
+        for (int q = pt.off[kb]; q < pt.off[kb + 1]; ++q)
+            if (tm[q].b == bit) f += tm[q].w * bit_spin(base, tm[q].a);
+        r.f[a] = f;
+        double sn, cs;
+        sincos(s_c * f, &sn, &cs);
+        r.g[a] = c2{cs, sn};
+        if (a < 6) {
+            sincos(s_out * f, &sn, &cs);
+            r.gl[a] = c2{cs, sn};
+        }
+    }
+    double v = 0.0;
+    for (int q = pt.off[kBB]; q < pt.off[kBB + 1]; ++q) v -= tm[q].w * bit_spin(base, tm[q].a) * bit_spin(base, tm[q].b);
+    for (int q = pt.off[kFB]; q < pt.off[kFB + 1]; ++q) v -= tm[q].w * bit_spin(base, tm[q].a);
+    r.cst = v;
+    r.pad = 0.0;
+    double sn, cs;
+    sincos(s_c * v, &sn, &cs);
+    r.cp = c2{cs, -sn};
+    rec[tile] = r;
+}
+
+// quantum.rs:196-203 on register amplitudes: (a, b) = (x, x|bit)
+__device__ inline void rotate_pair(c2& a, c2& b, double c, double s) {
+    const double ra = a.x, ia = a.y, rb = b.x, ib = b.y;
+    a = c2{c * ra - s * ib, c * ia + s * rb};
+    b = c2{c * rb - s * ia, c * ib + s * ra};
+}
+
+// Rotate the register bits selected by `mask4` (bit i of mask4 <-> register bit i).
+__device__ inline void rotate_regs(c2 (&v)[kRegs], unsigned mask4, double c, double s) {
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+        if (!((mask4 >> bit) & 1u)) continue;
+#pragma unroll
+        for (int r = 0; r < kRegs; ++r)
+            if (!(r & (1 << bit))) rotate_pair(v[r], v[r | (1 << bit)], c, s);
+    }
+}
+
+// Register layouts of a tile (lay0, lay1, lay2 above).
+enum { kLay0 = 0, kLay1 = 1, kLay2 = 2 };
+__device__ inline int lay_of(int which, int t, int r) {
+    return which == kLay0 ? lay0(t, r) : which == kLay1 ? lay1(t, r) : lay2(t, r);
+}
+
+// One full-tile exchange through LDS from layout FROM to layout TO; `mid` runs
+// between the two barriers (it sees every LDS write made before the exchange).
+template <int FROM, int TO, class Mid>
+__device__ inline void exchange(c2 (&v)[kRegs], c2* sh, Mid&& mid) {
     const int t = threadIdx.x;
-    auto lay = [&](int which, int r) { return which == 0 ? lay0(t, r) : which == 1 ? lay1(t, r) : lay2(t, r); };
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) sh[sidx(lay(FROM, r))] = v[r];
+    for (int r = 0; r < kRegs; ++r) sh[sidx(lay_of(FROM, t, r))] = v[r];
     __syncthreads();
+    mid();
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay(TO, r))];
+    for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay_of(TO, t, r))];
     __syncthreads();
 }
 
@@ -305,183 +416,158 @@ __device__ inline void exchange(double2 (&v)[kRegs], double2* sh) {
 // bits 8-11) through lay2 (bits 0-3) to lay1 (bits 4-7), or back from lay1 to
 // lay0. The qubits of a group commute; within a group the order differs from
 // the upstream ascending loop only by rounding.
-__device__ inline void rotate_group_fwd(double2 (&v)[kRegs], double2* sh, unsigned mask, double c, double s) {
+template <class Mid>
+__device__ inline void rotate_group_fwd(c2 (&v)[kRegs], c2* sh, unsigned mask, double c, double s, Mid&& mid) {
     rotate_regs(v, (mask >> 8) & 15u, c, s);
-    exchange<0, 2>(v, sh);
+    exchange<kLay0, kLay2>(v, sh, mid);
     rotate_regs(v, mask & 15u, c, s);
-    exchange<2, 1>(v, sh);
+    exchange<kLay2, kLay1>(v, sh, [] {});
     rotate_regs(v, (mask >> 4) & 15u, c, s);
 }
-__device__ inline void rotate_group_bwd(double2 (&v)[kRegs], double2* sh, unsigned mask, double c, double s) {
+__device__ inline void rotate_group_bwd(c2 (&v)[kRegs], c2* sh, unsigned mask, double c, double s) {
     rotate_regs(v, (mask >> 4) & 15u, c, s);
-    exchange<1, 2>(v, sh);
+    exchange<kLay1, kLay2>(v, sh, [] {});
     rotate_regs(v, mask & 15u, c, s);
-    exchange<2, 0>(v, sh);
+    exchange<kLay2, kLay0>(v, sh, [] {});
     rotate_regs(v, (mask >> 8) & 15u, c, s);
 }
 
-// PH: phase block (merged pass); RA / RB: rotate before / after it. The tile is
-// loaded in lay0; a rotation from lay0 ends in lay1 and one from lay1 in lay0,
-// so the phase layout and the store layout follow from RA and RB at compile time.
+// One streaming pass. PH: phase block (merged pass); RA / RB: rotate before /
+// after it; FAST: the phase block works on the registers from per-tile tables
+// (k_sv_tile_phases) and pass-constant coupling tables, else on the LDS copy of
+// the tile from tables built in the pass. The tile is loaded in lay0; a rotation
+// from lay0 ends in lay1 and one from lay1 in lay0, so the phase layout and the
+// store layout follow from RA and RB at compile time.
 template <bool PH, bool RA, bool RB, bool FAST>
 __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ psi, PassTerms pt, PassArgs pa,
-                                                      long long ntiles, double* __restrict__ partials,
-                                                      double* __restrict__ best_p, long long* __restrict__ best_x) {
-    // FAST: the phase block of the register-slot loop (pa.fast set by the host),
-    // else the LDS phase block with a per-coupling loop; one of them per instance
-    constexpr int cur = RA ? kLay1 : kLay0;                         // layout of the phase block
+                                                      long long ntiles, const TileRec* __restrict__ rec,
+                                                      double* __restrict__ partials, double* __restrict__ best_p,
+                                                      long long* __restrict__ best_x) {
+    constexpr int cur = RA ? kLay1 : kLay0;                             // layout of the phase block
     constexpr int out_lay = RB ? (cur == kLay1 ? kLay0 : kLay1) : cur;  // layout of the store
-    __shared__ double2 sh[kTile + kTile / 16];
-    __shared__ Term tm[kMaxTerms];
+    __shared__ c2 sh[kTile + kTile / 16];
+    __shared__ Term tm[FAST ? 1 : kMaxTerms];  // the LDS phase block's terms (FAST reads pt.t once)
     __shared__ int off[kKinds + 1];
     __shared__ double tabE[128];
-    __shared__ double2 tabPo[128], tabPi[128];
-    __shared__ double2 crossPo[kMaxCrossT], crossPi[kMaxCrossT];
+    __shared__ double2 tabPo[128], tabPi[FAST ? 64 : 128];
+    __shared__ double2 crossPo[kMaxCrossT], crossPi[FAST ? 1 : kMaxCrossT];
     __shared__ double cst;
     __shared__ double2 cstPo, cstPi;
     __shared__ double red[kThreads / 64][kObs + 1];
     __shared__ long long redx[kThreads / 64];
     __shared__ double xE[1 << kFastCross];
+    __shared__ c2 xCorr[1 << kFastCross];  // exp(-i s_out e(k)) conj(exp(-i s_out e(k ^ x0mask)))
+    __shared__ c2 shrec[2][kRecChunks];  // by tile parity: a pass without RB reads it after its last barrier
     const int t = threadIdx.x;
-    // fast path: the phase of one application covers phase_out and phase_in
+    // one combined phase covers phase_out and phase_in
     const double s_c = (pa.phase_out ? pa.scale_out : 0.0) + (pa.phase_in ? pa.scale_in : 0.0);
     if (PH) {
-        for (int q = t; q < pt.off[kKinds]; q += kThreads) tm[q] = pt.t[q];
+        if (!FAST)
+            for (int q = t; q < pt.off[kKinds]; q += kThreads) tm[q] = pt.t[q];
         if (t <= kKinds) off[t] = pt.off[t];
         if (FAST && t < (1 << pa.nx)) {
             // pass constants: energy and phasors of the L-H couplings per sign pattern
-            // (bit q of t set <-> the spins of coupling q differ: -w s s = +w)
-            double e = 0.0;
-            for (int q = 0; q < pa.nx; ++q) e += ((t >> q) & 1) ? pa.xw[q] : -pa.xw[q];
+            // (bit q of t set <-> the spins of coupling q differ: -w s s = +w), and the
+            // rho01 factor exp(-i s_out e(k)) conj(exp(-i s_out e(k ^ x0mask)))
+            auto ecross = [&](int k) {
+                double e = 0.0;
+                for (int q = 0; q < pa.nx; ++q) e += ((k >> q) & 1) ? pa.xw[q] : -pa.xw[q];
+                return e;
+            };
+            const double e = ecross(t), e1 = ecross(t ^ pa.x0mask);
             xE[t] = e;
-            double sn, cs;
+            double sn, cs, sn1, cs1;
             sincos(s_c * e, &sn, &cs);
             crossPo[t] = make_double2(cs, -sn);  // exp(-i s_c e)
             sincos(pa.scale_out * e, &sn, &cs);
-            crossPi[t] = make_double2(cs, -sn);  // exp(-i s_out e)
+            sincos(pa.scale_out * e1, &sn1, &cs1);
+            xCorr[t] = cm(c2{cs, -sn}, c2{cs1, sn1});
         }
         __syncthreads();
     }
-    // Fast path, pass constants kept in registers across tiles: thread t < 128 holds
-    // the base-independent part Tc of table entry t (LL edges and L fields for
-    // t < 64, HH edges and H fields for t >= 64) and its phasor exp(-i s_c Tc);
-    // threads 128..191 hold exp(-i s_out Tc) of the L entries; threads 0..11 hold
-    // the range of base couplings of local bit t (BL / BH terms sorted by local bit).
+    // FAST: pass constants kept in registers across tiles: thread t < 128 holds the
+    // base-independent part Tc of table entry t (LL edges and L fields for t < 64,
+    // HH edges and H fields for t >= 64) and its phasor exp(-i s_c Tc); threads
+    // 128..191 hold exp(-i s_out Tc) of the L entries.
     double tc = 0.0;
-    double2 pcc = make_double2(1.0, 0.0);
-    int fb0 = 0, fb1 = 0;
-    if (PH && FAST) {
-        if (t < 192) {
-            const int u = t < 128 ? (t & 63) : t - 128;
-            const bool lowh = t < 64 || t >= 128;
-            const int kp = lowh ? kLL : kHH, kf = lowh ? kFL : kFH;
-            for (int q = off[kp]; q < off[kp + 1]; ++q) tc -= tm[q].w * bit_spin(u, tm[q].a) * bit_spin(u, tm[q].b);
-            for (int q = off[kf]; q < off[kf + 1]; ++q) tc -= tm[q].w * bit_spin(u, tm[q].a);
-            double sn, cs;
-            sincos((t < 128 ? s_c : pa.scale_out) * tc, &sn, &cs);
-            pcc = make_double2(cs, -sn);
-        }
-        if (t < 12) {
-            const int kb = t < 6 ? kBL : kBH, bit = t % 6;
-            fb0 = off[kb + 1];
-            fb1 = off[kb + 1];
-            for (int q = off[kb]; q < off[kb + 1]; ++q)
-                if (tm[q].b == bit) {
-                    fb0 = min(fb0, q);
-                    fb1 = q + 1;
-                }
-        }
+    c2 pcc{1.0, 0.0};
+    if (PH && FAST && t < 192) {
+        const int u = t < 128 ? (t & 63) : t - 128;
+        const bool lowh = t < 64 || t >= 128;
+        const int kp = lowh ? kLL : kHH, kf = lowh ? kFL : kFH;
+        const Term* T = pt.t;
+        for (int q = pt.off[kp]; q < pt.off[kp + 1]; ++q) tc -= T[q].w * bit_spin(u, T[q].a) * bit_spin(u, T[q].b);
+        for (int q = pt.off[kf]; q < pt.off[kf + 1]; ++q) tc -= T[q].w * bit_spin(u, T[q].a);
+        double sn, cs;
+        sincos((t < 128 ? s_c : pa.scale_out) * tc, &sn, &cs);
+        pcc = c2{cs, -sn};
     }
     const int c = pa.c, lo = pa.lo;
-    const int lo_span = lo - c;  // tile-id bits placed in [c, lo)
-    const unsigned long long cmask = (1ull << c) - 1ull;
+    // per-thread parts of the load and store addresses (the register parts are uniform)
+    const unsigned long long toff_ld = scat((unsigned)lay_of(kLay0, t, 0), c, lo);
+    const unsigned long long toff_st = scat((unsigned)lay_of(out_lay, t, 0), c, lo);
     double acc[kObs] = {0, 0, 0, 0, 0, 0};
     Best best{-1.0, 0};
 
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        unsigned long long base;
-        if (lo < 0) {
-            base = (unsigned long long)tile << kTileBits;
-        } else {
-            const unsigned long long tl = (unsigned long long)tile & ((1ull << lo_span) - 1ull);
-            const unsigned long long th = (unsigned long long)tile >> lo_span;
-            base = (tl << c) | (th << (lo + (kTileBits - c)));
+    auto load_tile = [&](c2 (&w)[kRegs], c2& wr, long long tile) __attribute__((always_inline)) {
+        const double2* src = psi + tile_base(tile, c, lo) + toff_ld;
+#pragma unroll
+        for (int r = 0; r < kRegs; ++r) w[r] = ldc(src + scat((unsigned)lay_of(kLay0, 0, r), c, lo));
+        // every thread loads a piece of the record (threads >= kRecChunks repeat the
+        // last one) so that no load sits under a divergent branch
+        if (PH && FAST) wr = ldc(reinterpret_cast<const double2*>(rec + tile) + min(t, kRecChunks - 1));
+    };
+
+    // FAST table build from the tile record (threads < 192, after the record is in LDS)
+    auto build_tables = [&](int par) __attribute__((always_inline)) {
+        const TileRec* R = reinterpret_cast<const TileRec*>(shrec[par]);
+        if (t < 128) {
+            const int u = t & 63, half = t < 64 ? 0 : 6;
+            double e = tc;
+            c2 ph = pcc;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                const bool up = (u >> a) & 1;
+                e -= up ? R->f[half + a] : -R->f[half + a];
+                ph = cm(ph, up ? R->g[half + a] : cj(R->g[half + a]));
+            }
+            tabE[t] = e;
+            tabPo[t] = make_double2(ph.x, ph.y);  // exp(-i s_c T)
+        } else if (t < 192) {
+            const int u = t - 128;
+            c2 ph = pcc;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) ph = cm(ph, ((u >> a) & 1) ? R->gl[a] : cj(R->gl[a]));
+            tabPi[u] = make_double2(ph.x, ph.y);  // exp(-i s_out T1)
         }
-        auto gidx = [&](int j) -> unsigned long long {
-            return lo < 0 ? (base | (unsigned long long)j)
-                          : (base | ((unsigned long long)j & cmask) | (((unsigned long long)j >> c) << lo));
-        };
-        double2 v[kRegs];
-#pragma unroll
-        for (int r = 0; r < kRegs; ++r) v[r] = psi[gidx(lay0(t, r))];  // loads in flight during the tables
+    };
+
+    // One tile, loaded into v (and its record piece into vr); `par` alternates
+    // between consecutive tiles of the workgroup.
+    auto body = [&](c2 (&v)[kRegs], c2& vr, long long tile, int par) __attribute__((always_inline)) {
+        const unsigned long long base = tile_base(tile, c, lo);
         if (PH && FAST) {
-            // The tile's base enters only through the base-local couplings, i.e. a
-            // field f_a = sum_b w s_b(base) on local bit a: T[j] = Tc[j] - sum_a f_a
-            // s_a(j), and exp(-i s T) = exp(-i s Tc) * prod_a exp(+i s f_a s_a), so a
-            // tile needs 19 sincos (fields, constant) instead of one per entry.
-            // Scratch in tabPi[64..127] (unused by the fast path): f at 64..75,
-            // exp(i s_c f) at 76..87, exp(i s_out f) of the L bits at 88..93.
-            double2* fscr = tabPi + 64;
-            if (t < 12) {
-                double f = 0.0;
-                for (int q = fb0; q < fb1; ++q)
-                    if (tm[q].b == t % 6) f += tm[q].w * bit_spin(base, tm[q].a);
-                fscr[t].x = f;
-            } else if (t >= 128 && t < 192) {  // C: BB edges and B fields
-                const int lane = t & 63;
-                double v0 = 0.0;
-                for (int q = off[kBB] + lane; q < off[kBB + 1]; q += 64)
-                    v0 -= tm[q].w * bit_spin(base, tm[q].a) * bit_spin(base, tm[q].b);
-                for (int q = off[kFB] + lane; q < off[kFB + 1]; q += 64) v0 -= tm[q].w * bit_spin(base, tm[q].a);
-                v0 = wave_sum(v0);
-                if (lane == 0) cst = v0;
+            if (t < kRecChunks) shrec[par][t] = vr;
+            if (!RA) {
+                __syncthreads();
+                build_tables(par);
+                __syncthreads();
             }
-            __syncthreads();
-            {
-                double sn, cs;
-                if (t < 12) {
-                    sincos(s_c * fscr[t].x, &sn, &cs);
-                    fscr[12 + t] = make_double2(cs, sn);
-                } else if (t < 18) {
-                    sincos(pa.scale_out * fscr[t - 12].x, &sn, &cs);
-                    fscr[24 + t - 12] = make_double2(cs, sn);
-                } else if (t == 192) {
-                    sincos(s_c * cst, &sn, &cs);
-                    cstPo = make_double2(cs, -sn);
-                }
-                if (t < 128) {
-                    const int u = t & 63, half = t < 64 ? 0 : 6;
-                    double e = tc;
-#pragma unroll
-                    for (int a = 0; a < 6; ++a) e -= fscr[half + a].x * bit_spin(u, a);
-                    tabE[t] = e;
-                }
-            }
-            __syncthreads();
-            if (t < 192) {
-                const int u = t < 128 ? (t & 63) : t - 128;
-                const double2* g = t < 128 ? fscr + 12 + (t < 64 ? 0 : 6) : fscr + 24;
-                double2 ph = pcc;
-#pragma unroll
-                for (int a = 0; a < 6; ++a) ph = cmul(ph, ((u >> a) & 1) ? g[a] : cconj(g[a]));
-                if (t < 128)
-                    tabPo[t] = ph;  // exp(-i s_c T)
-                else
-                    tabPi[t - 128] = ph;  // exp(-i s_out T1)
-            }
-            __syncthreads();
         }
         if (PH && !FAST) {
             build_energy(tm, off, base, tabE, &cst);
             __syncthreads();
-            {
-                if (pa.phase_out) build_phasors(tm, off, pa.scale_out, tabE, cst, tabPo, crossPo, &cstPo);
-                if (pa.phase_in) build_phasors(tm, off, pa.scale_in, tabE, cst, tabPi, crossPi, &cstPi);
-            }
+            if (pa.phase_out) build_phasors(tm, off, pa.scale_out, tabE, cst, tabPo, crossPo, &cstPo);
+            if (pa.phase_in) build_phasors(tm, off, pa.scale_in, tabE, cst, tabPi, crossPi, &cstPi);
             __syncthreads();
         }
-        if (RA) rotate_group_fwd(v, sh, pa.mask, pa.ca, pa.sa);  // lay0 -> lay1
-        if (PH && FAST) {
+        if (RA) {
+            if (PH && FAST)
+                rotate_group_fwd(v, sh, pa.mask, pa.ca, pa.sa, [&] { build_tables(par); });  // lay0 -> lay1
+            else
+                rotate_group_fwd(v, sh, pa.mask, pa.ca, pa.sa, [] {});
+        }
+        if (PH && FAST && cur != kLay1) {
             // phase block, one sweep: E(j) = C + T1[j & 63] + T2[j >> 6] + xE[k(j)]
             // with k(j) the sign pattern of the L-H couplings; one phasor
             // exp(-i s_c E) applies phase_out and phase_in together. rho01 pairs
@@ -490,42 +576,106 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
             // Po(j) conj(Po(j ^ 1)), in which C and T2 cancel:
             // P1o[lo] conj(P1o[lo ^ 1]) xPo[k] conj(xPo[k ^ x0mask]). The tile goes
             // through each thread's own LDS slots (no barrier: a thread reads back
-            // only what it wrote) so the loop need not be unrolled.
+            // only what it wrote, and the next exchange writes the same slots) so
+            // the loop is unrolled only four ways. (This form serves the first pass
+            // of an anneal, in lay0; the passes in lay1 take the block below.)
+            const TileRec* R = reinterpret_cast<const TileRec*>(shrec[par]);
+            const double cstv = R->cst;
+            const c2 cp = R->cp;
             const bool even = (t & 1) == 0;
 #pragma unroll
             for (int r = 0; r < kRegs; ++r) sh[sidx(lay_of(cur, t, r))] = v[r];
-#pragma unroll 1
+#pragma unroll 4
             for (int r = 0; r < kRegs; ++r) {
                 const int j = lay_of(cur, t, r);
                 int k = 0;
 #pragma unroll
                 for (int q = 0; q < kFastCross; ++q)
                     if (q < pa.nx) k |= (((j >> pa.xa[q]) ^ (j >> pa.xb[q])) & 1) << q;
-                const int lo = j & 63, hi = j >> 6;
-                const double2 u = sh[sidx(j)];
+                const int jl = j & 63, jh = j >> 6;
+                const c2 u = sh[sidx(j)];
                 if (pa.phase_out) {
                     const double p = u.x * u.x + u.y * u.y;
-                    const double e = cst + tabE[lo] + tabE[64 + hi] + xE[k];
+                    const double e = cstv + tabE[jl] + tabE[64 + jh] + xE[k];
                     acc[4] += p;
                     acc[5] += p * e;
-                    if (pa.best) best = better(best, Best{p, (long long)gidx(j)});
-                    const double2 w = make_double2(__shfl_xor(u.x, 1, 64), __shfl_xor(u.y, 1, 64));
+                    if (pa.best) best = better(best, Best{p, (long long)(base + scat((unsigned)j, c, lo))});
+                    const c2 wv{__shfl_xor(u.x, 1, 64), __shfl_xor(u.y, 1, 64)};
                     if (even) {
-                        const double2 r1 = cmul(tabPi[lo], cconj(tabPi[lo ^ 1]));
-                        const double2 corr = cmul(r1, cmul(crossPi[k], cconj(crossPi[k ^ pa.x0mask])));
-                        const double2 uw = make_double2(u.x * w.x + u.y * w.y, u.y * w.x - u.x * w.y);
-                        const double2 z = cmul(uw, corr);
+                        const c2 r1 = cm(c2of(tabPi[jl]), cj(c2of(tabPi[jl ^ 1])));
+                        const c2 corr = cm(r1, xCorr[k]);
+                        const c2 uw{u.x * wv.x + u.y * wv.y, u.y * wv.x - u.x * wv.y};
+                        const c2 z = cm(uw, corr);
                         acc[0] += p;
-                        acc[1] += w.x * w.x + w.y * w.y;
+                        acc[1] += wv.x * wv.x + wv.y * wv.y;
                         acc[2] += z.x;
                         acc[3] += z.y;
                     }
                 }
-                const double2 ph = cmul(cmul(cstPo, tabPo[lo]), cmul(tabPo[64 + hi], crossPo[k]));
-                sh[sidx(j)] = cmul(u, ph);
+                const c2 ph = cm(cm(cp, c2of(tabPo[jl])), cm(c2of(tabPo[64 + jh]), c2of(crossPo[k])));
+                sh[sidx(j)] = cm(u, ph);
             }
 #pragma unroll
             for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay_of(cur, t, r))];
+        }
+        if (PH && FAST && cur == kLay1) {
+            // The same phase block in lay1, j = (t & 15) | (r << 4) | ((t >> 4) << 8):
+            // with r = 4b + a the L index jl = (t & 15) | (a << 4) takes four values
+            // per thread and the H index jh = b | ((t >> 4) << 2) four more, so their
+            // table entries (energy, combined phasor, rho01 factor of the L part) are
+            // read once per tile; the coupling part comes from one pass-constant
+            // entry per sign pattern. The rho01 partner j ^ 1 is lane t ^ 1, read
+            // with a quad permute.
+            const TileRec* R = reinterpret_cast<const TileRec*>(shrec[par]);
+            const double cstv = R->cst;
+            const c2 cp = R->cp;
+            const bool even = (t & 1) == 0;
+            double EL[4];
+            c2 PL[4], R1[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int jl = (t & 15) | (a << 4);
+                EL[a] = cstv + tabE[jl];
+                PL[a] = cm(cp, c2of(tabPo[jl]));
+                R1[a] = cm(c2of(tabPi[jl]), cj(c2of(tabPi[jl ^ 1])));
+            }
+#pragma unroll
+            for (int r = 0; r < kRegs; ++r) sh[sidx(lay_of(kLay1, t, r))] = v[r];
+#pragma unroll 1
+            for (int b = 0; b < 4; ++b) {
+                const int jh = b | ((t >> 4) << 2);
+                const double eh = tabE[64 + jh];
+                const c2 phb = c2of(tabPo[64 + jh]);
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int j = lay_of(kLay1, t, 4 * b + a);
+                    int k = 0;
+#pragma unroll
+                    for (int q = 0; q < kFastCross; ++q)
+                        if (q < pa.nx) k |= (((j >> pa.xa[q]) ^ (j >> pa.xb[q])) & 1) << q;
+                    const c2 u = sh[sidx(j)];
+                    if (pa.phase_out) {
+                        const double p = u.x * u.x + u.y * u.y;
+                        const double e = EL[a] + eh + xE[k];
+                        acc[4] += p;
+                        acc[5] += p * e;
+                        if (pa.best) best = better(best, Best{p, (long long)(base + scat((unsigned)j, c, lo))});
+                        const c2 wv{quad_xor1(u.x), quad_xor1(u.y)};
+                        if (even) {
+                            const c2 corr = cm(R1[a], xCorr[k]);
+                            const c2 uw{u.x * wv.x + u.y * wv.y, u.y * wv.x - u.x * wv.y};
+                            const c2 z = cm(uw, corr);
+                            acc[0] += p;
+                            acc[1] += wv.x * wv.x + wv.y * wv.y;
+                            acc[2] += z.x;
+                            acc[3] += z.y;
+                        }
+                    }
+                    sh[sidx(j)] = cm(u, cm(cm(PL[a], phb), c2of(crossPo[k])));
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kRegs; ++r) v[r] = sh[sidx(lay_of(kLay1, t, r))];
         }
         if (PH && !FAST) {
             // phase block on the LDS copy of the tile (lay0 order), plain loops keep registers low
@@ -534,31 +684,35 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
             __syncthreads();
             if (pa.phase_out) {
 #pragma unroll 1
-                for (int j = t; j < kTile; j += kThreads)
-                    sh[sidx(j)] = cmul(sh[sidx(j)], tile_phasor(tm, off, j, tabPo, crossPo, cstPo));
+                for (int j = t; j < kTile; j += kThreads) {
+                    const double2 f = tile_phasor(tm, off, j, tabPo, crossPo, cstPo);
+                    sh[sidx(j)] = cm(sh[sidx(j)], c2of(f));
+                }
                 __syncthreads();
 #pragma unroll 1
                 for (int j = t; j < kTile; j += kThreads) {
-                    const double2 u = sh[sidx(j)];
+                    const c2 u = sh[sidx(j)];
                     const double e = tile_energy(tm, off, j, tabE, cst);
                     const double p = u.x * u.x + u.y * u.y;
                     acc[4] += p;
                     acc[5] += p * e;
-                    if (pa.best) best = better(best, Best{p, (long long)gidx(j)});
+                    if (pa.best) best = better(best, Best{p, (long long)(base + scat((unsigned)j, c, lo))});
                     if ((j & 1) == 0) {  // local bit 0 == global bit 0
-                        const double2 w = sh[sidx(j + 1)];
+                        const c2 w2 = sh[sidx(j + 1)];
                         acc[0] += u.x * u.x + u.y * u.y;
-                        acc[1] += w.x * w.x + w.y * w.y;
-                        acc[2] += u.x * w.x + u.y * w.y;
-                        acc[3] += u.y * w.x - u.x * w.y;
+                        acc[1] += w2.x * w2.x + w2.y * w2.y;
+                        acc[2] += u.x * w2.x + u.y * w2.y;
+                        acc[3] += u.y * w2.x - u.x * w2.y;
                     }
                 }
                 __syncthreads();
             }
             if (pa.phase_in) {
 #pragma unroll 1
-                for (int j = t; j < kTile; j += kThreads)
-                    sh[sidx(j)] = cmul(sh[sidx(j)], tile_phasor(tm, off, j, tabPi, crossPi, cstPi));
+                for (int j = t; j < kTile; j += kThreads) {
+                    const double2 f = tile_phasor(tm, off, j, tabPi, crossPi, cstPi);
+                    sh[sidx(j)] = cm(sh[sidx(j)], c2of(f));
+                }
                 __syncthreads();
             }
 #pragma unroll
@@ -569,11 +723,20 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
             if (cur == kLay1)
                 rotate_group_bwd(v, sh, pa.mask, pa.cb, pa.sb);
             else
-                rotate_group_fwd(v, sh, pa.mask, pa.cb, pa.sb);
+                rotate_group_fwd(v, sh, pa.mask, pa.cb, pa.sb, [] {});
         }
+        double2* dst = psi + base + toff_st;
 #pragma unroll
-        for (int r = 0; r < kRegs; ++r) psi[gidx(lay_of(out_lay, t, r))] = v[r];
-        if (PH) __syncthreads();  // tables are rebuilt for the next tile
+        for (int r = 0; r < kRegs; ++r) stc(dst + scat((unsigned)lay_of(out_lay, 0, r), c, lo), v[r]);
+        if (PH && !FAST) __syncthreads();  // tables are rebuilt for the next tile
+    };
+
+    int par = 0;
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x, par ^= 1) {
+        c2 v[kRegs];
+        c2 vr{0.0, 0.0};
+        load_tile(v, vr, tile);
+        body(v, vr, tile, par);
     }
     if (PH && pa.phase_out) {
 #pragma unroll
@@ -766,6 +929,8 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state, DevArena* arena
         }
     } else {
         const long long ntiles = dim >> kTileBits;
+        // 4096 workgroups of 64 tiles each at n = 30 (measured: one resident grid of two
+        // workgroups per CU streams 10% slower, 16384 workgroups the same)
         const int nblocks = (int)std::min<long long>(ntiles, 4096);
         const bool fast_phase = true;  // register phase block wherever the L-H couplings fit
         double* partials = ar.take<double>((std::size_t)nblocks * kObs);
@@ -834,6 +999,7 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state, DevArena* arena
             pts[g].off[kKinds] = acc;
             if (acc > kMaxTerms) throw Error(-1, "more than 256 energy terms (m + n) per tile");
         }
+        TileRec* d_rec = ar.take<TileRec>((std::size_t)ntiles);
         Term* d_terms = ar.take<Term>(std::max<std::size_t>(terms.size(), 1));
         if (!terms.empty())
             TQ_HIP(hipMemcpy(d_terms, terms.data(), terms.size() * sizeof(Term), hipMemcpyHostToDevice));
@@ -843,10 +1009,18 @@ SvOutcome sv_anneal_device(const SvProblem& pb, bool keep_state, DevArena* arena
             // the passes the host loop below issues: rotate only; phase then rotate (first);
             // rotate, phase, rotate (step boundary); rotate then phase (last)
             auto go = [&](auto kern) {
-                hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kThreads), 0, st, psi, pts[g], pa, ntiles, partials,
-                                   bp, bx);
+                hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kThreads), 0, st, psi, pts[g], pa, ntiles, d_rec,
+                                   partials, bp, bx);
             };
             const bool f = pa.fast != 0;
+            if (ph && f) {
+                // the tile records of this pass (fields and phasors of every tile)
+                const double s_c = (pa.phase_out ? pa.scale_out : 0.0) + (pa.phase_in ? pa.scale_in : 0.0);
+                const int rb = (int)((ntiles + 255) / 256);
+                hipLaunchKernelGGL(k_sv_tile_phases, dim3(rb), dim3(256), 0, st, pts[g], pa.c, pa.lo, ntiles, s_c,
+                                   pa.scale_out, d_rec);
+                TQ_LAUNCH_CHECK();
+            }
             if (!ph && pa.rot_a && !pa.rot_b) go(k_sv_pass<false, true, false, false>);
             else if (ph && !pa.rot_a && pa.rot_b) f ? go(k_sv_pass<true, false, true, true>) : go(k_sv_pass<true, false, true, false>);
             else if (ph && pa.rot_a && pa.rot_b) f ? go(k_sv_pass<true, true, true, true>) : go(k_sv_pass<true, true, true, false>);

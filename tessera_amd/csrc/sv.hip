@@ -630,6 +630,16 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
             const double cstv = R->cst;
             const c2 cp = R->cp;
             const bool even = (t & 1) == 0;
+            // k(j) = k(T) ^ k(R) for j = T | R with disjoint bits: the thread part once,
+            // the register part (uniform) per slot
+            auto kpat = [&](int j) {
+                int k = 0;
+#pragma unroll
+                for (int q = 0; q < kFastCross; ++q)
+                    if (q < pa.nx) k |= (((j >> pa.xa[q]) ^ (j >> pa.xb[q])) & 1) << q;
+                return k;
+            };
+            const int kt = kpat(lay_of(kLay1, t, 0));
             double EL[4];
             c2 PL[4], R1[4];
 #pragma unroll
@@ -649,10 +659,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_pass(double2* __restrict__ p
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
                     const int j = lay_of(kLay1, t, 4 * b + a);
-                    int k = 0;
-#pragma unroll
-                    for (int q = 0; q < kFastCross; ++q)
-                        if (q < pa.nx) k |= (((j >> pa.xa[q]) ^ (j >> pa.xb[q])) & 1) << q;
+                    const int k = kt ^ kpat(lay_of(kLay1, 0, 4 * b + a));
                     const c2 u = sh[sidx(j)];
                     if (pa.phase_out) {
                         const double p = u.x * u.x + u.y * u.y;

@@ -485,155 +485,6 @@ struct SvdShared {
     double beta;
 };
 
-// ---- LQ preconditioning of the block-Jacobi SVD ----------------------------
-// W (m x c, m >= c, column-major) <- W Q = L, lower trapezoidal, by Householder
-// reflectors applied from the right (reflector k zeroes row k beyond column k).
-// L has the left singular vectors and singular values of W, and the one-sided
-// Jacobi converges on L's columns in about half the sweeps it needs on W's
-// (the rows of a QR/LQ factor are graded; on the thetas of a saturated run 10-14
-// sweeps instead of 22-43 in a CPU study). Rows go in tiles of 16; tile t
-// belongs to worker t mod nworkers. Panel q (rows 16q..16q+15) is factorised by
-// its owner in LDS, then every worker applies the panel's 16 reflectors to its
-// later tiles: one grid barrier per panel. Reflector k is kept as column k of
-// `aux` (entries at rows >= k, v_k = 1) with tau in taus[k].
-// Tile in LDS: element (row rr, column r0 + jj) at lds[jj * 16 + rr]; thread
-// (rr = tid & 15, g = tid >> 4) owns row rr, columns jj = g, g + 32, ...
-template <bool XL>
-__device__ __forceinline__ void lq_load_tile(const double2* __restrict__ Wg, int ld, double2* lds, int t, int r0,
-                                             int n, int m) {
-    // four loads in flight per thread before their LDS stores (a load-store
-    // loop would wait out each load in turn)
-    constexpr int kBatch = 4;
-    const int total = n * 16;
-    for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
-        double2 v[kBatch];
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const int idx = base + q * blockDim.x;
-            const int jj = idx >> 4, rr = idx & 15, row = t * 16 + rr;
-            v[q] = (idx < total && row < m) ? gld<XL>(Wg + (std::size_t)(r0 + jj) * ld + row) : make_double2(0, 0);
-        }
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const int idx = base + q * blockDim.x;
-            if (idx < total) lds[idx] = v[q];
-        }
-    }
-    __syncthreads();
-}
-__device__ __forceinline__ void lq_store_tile(double2* __restrict__ Wg, int ld, const double2* lds, int t, int r0,
-                                              int n, int m) {
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < n * 16; idx += blockDim.x) {
-        const int jj = idx >> 4, rr = idx & 15, row = t * 16 + rr;
-        if (row < m) Wg[(std::size_t)(r0 + jj) * ld + row] = lds[idx];
-    }
-}
-// sum over the 32 column groups of each tile row: in-wave over the 4 lanes of
-// a row, then over the waves through LDS; returns the row's total to all its threads.
-__device__ __forceinline__ double2 lq_row_sum(double2 v, SvdShared& sh) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, rr = tid & 15;
-    v.x += __shfl_xor(v.x, 16, 64);
-    v.y += __shfl_xor(v.y, 16, 64);
-    v.x += __shfl_xor(v.x, 32, 64);
-    v.y += __shfl_xor(v.y, 32, 64);
-    if (lane < 16) sh.red[wave][rr] = v;
-    __syncthreads();
-    double2 tot = make_double2(0, 0);
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot = cadd(tot, sh.red[w][rr]);
-    __syncthreads();
-    return tot;
-}
-// tile rows rr > rmin: row <- row (I - tau v v^H), reflector from column i0 on;
-// reflector entry jj at v[jj * vs].
-template <bool GV>  // GV: v is in global memory, written by another workgroup of an XCD-local group
-__device__ __forceinline__ void lq_apply(double2* lds, int n, int i0, const double2* v, int vs, double2 tau,
-                                         int rmin, SvdShared& sh) {
-    const int rr = threadIdx.x & 15, g = threadIdx.x >> 4, ng = blockDim.x >> 4;
-    const int j0 = i0 + ((g - i0 % ng + ng) % ng);  // first jj >= i0 with jj = g (mod ng)
-    double2 w = make_double2(0, 0);
-    // unrolled so that several reflector loads are in flight (the additions keep their order)
-#pragma unroll 2
-    for (int jj = j0; jj < n; jj += ng) w = cadd(w, cmul(lds[jj * 16 + rr], gld<GV>(v + (std::size_t)jj * vs)));
-    w = lq_row_sum(w, sh);
-    if (rr > rmin) {
-        const double2 tw = cmul(tau, w);
-        for (int jj = j0; jj < n; jj += ng)
-            lds[jj * 16 + rr] = csub(lds[jj * 16 + rr], cmulc(gld<GV>(v + (std::size_t)jj * vs), tw));  // - tau w conj(v)
-    }
-    __syncthreads();
-}
-
-template <class Sync, bool XL>
-__device__ void lq_precondition(double2* __restrict__ Wg, int ld, int m, int c, double2* __restrict__ aux,
-                                double2* __restrict__ taus, int me, int nworkers, Sync& sync, double2* lds,
-                                SvdShared& sh, int panels, bool active) {
-    const int tid = threadIdx.x, rr = tid & 15, g = tid >> 4, ng = blockDim.x >> 4;
-    const int ntiles = (m + 15) / 16;
-    for (int q = 0; q < panels; ++q) {
-        const int r0 = q * 16;
-        const bool live = active && r0 < c;
-        const int n = c - r0, nb = min(16, c - r0);
-        if (live && q % nworkers == me) {  // factorise panel q
-            lq_load_tile<XL>(Wg, ld, lds, q, r0, n, m);
-            for (int i = 0; i < nb; ++i) {
-                const int k = r0 + i;
-                double2* v = aux + (std::size_t)k * ld + r0;  // v[jj] = reflector entry at column r0 + jj
-                // |row i beyond column i|^2: every thread sums row i over its column
-                // group, so each tile row's total below is the same number
-                double2 part = make_double2(0, 0);
-                for (int jj = i + 1 + ((g - (i + 1) % ng + ng) % ng); jj < n; jj += ng) part.x += cnorm(lds[jj * 16 + i]);
-                const double xn2 = lq_row_sum(part, sh).x;
-                if (tid == 0) {
-                    const double2 alpha = cconj(lds[i * 16 + i]);
-                    if (xn2 == 0.0 && alpha.y == 0.0) {
-                        sh.hh[0] = make_double2(0, 0);
-                        sh.hh[1] = make_double2(0, 0);
-                        sh.beta = alpha.x;
-                    } else {
-                        const double beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
-                        sh.hh[0] = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-                        const double2 d = make_double2(alpha.x - beta, alpha.y);  // 1 / (alpha - beta)
-                        const double dd = d.x * d.x + d.y * d.y;
-                        sh.hh[1] = make_double2(d.x / dd, -d.y / dd);
-                        sh.beta = beta;
-                    }
-                }
-                __syncthreads();
-                const double2 tau = sh.hh[0], scale = sh.hh[1];
-                // the reflector goes to aux (for the other tiles) and into tile row i,
-                // whose entries it replaces, for this tile's remaining rows
-                for (int jj = i + tid; jj < n; jj += blockDim.x) {
-                    const double2 e = jj == i ? make_double2(1, 0) : cmul(cconj(lds[jj * 16 + i]), scale);
-                    v[jj] = e;
-                    lds[jj * 16 + i] = e;
-                }
-                if (tid == 0) taus[k] = tau;
-                __syncthreads();
-                lq_apply<false>(lds, n, i, lds + i, 16, tau, i, sh);
-                for (int jj = i + tid; jj < n; jj += blockDim.x)
-                    lds[jj * 16 + i] = jj == i ? make_double2(sh.beta, 0) : make_double2(0, 0);
-                __syncthreads();
-            }
-            lq_store_tile(Wg, ld, lds, q, r0, n, m);
-        }
-        sync();
-        if (live)
-            for (int t = q + 1; t < ntiles; ++t) {
-                if (t % nworkers != me) continue;
-                lq_load_tile<XL>(Wg, ld, lds, t, r0, n, m);
-                for (int i = 0; i < nb; ++i) {
-                    const int k = r0 + i;
-                    lq_apply<XL>(lds, n, i, aux + (std::size_t)k * ld + r0, 1, gld<XL>(taus + k), -1, sh);
-                }
-                lq_store_tile(Wg, ld, lds, t, r0, n, m);
-            }
-    }
-    sync();
-}
-
-
-
 // |x|^2 over the lane-held rows, summed over the wave.
 __device__ inline double col_norm2(const double2 (&x)[kRowsPerLane]) {
     double a = 0.0;
@@ -955,6 +806,182 @@ __device__ __forceinline__ int round_rows(double2* __restrict__ Wg, int ld, int 
     return flags;
 }
 
+// ---- LQ preconditioning of the block-Jacobi SVD ----------------------------
+// W (m x c, m >= c, column-major) <- W Q = L, lower trapezoidal, by Householder
+// reflectors applied from the right (reflector k zeroes row k beyond column k).
+// L has the left singular vectors and singular values of W, and the one-sided
+// Jacobi converges on L's columns in about half the sweeps it needs on W's
+// (the rows of a QR/LQ factor are graded; on the thetas of a saturated run 10-14
+// sweeps instead of 22-43 in a CPU study). Rows go in tiles of 16; tile t
+// belongs to worker t mod nworkers, and inside a workgroup each tile row belongs
+// to one 32-lane team that holds it in registers (lane l: columns r0 + l + 32 u).
+// Panel q (rows 16q..16q+15, columns r0 = 16q onward) is factorised by its
+// owner: the team of row i builds reflector i from its own row and writes it to
+// LDS, one workgroup barrier publishes it, and every later row's team applies
+// it to its row (a half-wave reduction, no barrier). Reflector k (v_k = 1,
+// entries at columns >= k) is kept as column k of `aux` with tau in taus[k].
+// After one group barrier per panel every worker stages the panel's reflectors
+// in LDS once and each team applies all of them to its row of every later tile
+// it owns, without further barriers.
+// Reflectors in LDS: entry jj of reflector i at vl[i * S + jj], S = c + 1.
+constexpr int kLqCols = kMaxCols / 32;  // register slots of a team lane (columns per lane)
+
+__host__ __device__ constexpr std::size_t lq_lds_bytes(int c) {
+    return (std::size_t)16 * (c + 1) * sizeof(double2) + 16 * sizeof(double2);
+}
+
+// sum over the 32 lanes of this lane's half of the wave: the mirror levels leave
+// every lane of a row of 16 with the row's total, the permlane swap adds the
+// other row's
+__device__ __forceinline__ double half_sum(double v) {
+    v += dpp_f64<0x140>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0xB1>(v);
+    double a = v, b = v;
+    swap16(a, b);
+    return v + ((threadIdx.x & 16) ? a : b);
+}
+
+// row `row` of W, columns r0 + l32 + 32 u (u < kLqCols, inside r0..r0+n-1) into x
+template <bool XL>
+__device__ __forceinline__ void lq_load_row(const double2* __restrict__ Wg, int ld, int row, int r0, int n, int m,
+                                            int l32, double2 (&x)[kLqCols]) {
+#pragma unroll
+    for (int u = 0; u < kLqCols; ++u) {
+        const int jj = l32 + 32 * u;
+        x[u] = (jj < n && row < m) ? gld<XL>(Wg + (std::size_t)(r0 + jj) * ld + row) : make_double2(0.0, 0.0);
+    }
+}
+__device__ __forceinline__ void lq_store_row(double2* __restrict__ Wg, int ld, int row, int r0, int n, int m,
+                                             int l32, const double2 (&x)[kLqCols]) {
+#pragma unroll
+    for (int u = 0; u < kLqCols; ++u) {
+        const int jj = l32 + 32 * u;
+        if (jj < n && row < m) Wg[(std::size_t)(r0 + jj) * ld + row] = x[u];
+    }
+}
+// x <- x (I - tau v v^H) for reflector v at vl (entries from column i on)
+__device__ __forceinline__ void lq_reflect(double2 (&x)[kLqCols], const double2* vl, int i, int n, int l32,
+                                           double2 tau) {
+    double2 w = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < kLqCols; ++u) {
+        const int jj = l32 + 32 * u;
+        if (jj >= i && jj < n) w = cadd(w, cmul(x[u], vl[jj]));
+    }
+    w.x = half_sum(w.x);
+    w.y = half_sum(w.y);
+    const double2 tw = cmul(tau, w);
+#pragma unroll
+    for (int u = 0; u < kLqCols; ++u) {
+        const int jj = l32 + 32 * u;
+        if (jj >= i && jj < n) x[u] = csub(x[u], cmulc(vl[jj], tw));
+    }
+}
+
+template <class Sync, bool XL>
+__device__ void lq_precondition(double2* __restrict__ Wg, int ld, int m, int c, double2* __restrict__ aux,
+                                double2* __restrict__ taus, int me, int nworkers, Sync& sync, double2* lds,
+                                int panels, bool active) {
+    const int tid = threadIdx.x, team = tid >> 5, l32 = tid & 31;
+    const int S = c + 1;
+    double2* vl = lds;
+    double2* tauv = lds + (std::size_t)16 * S;  // taus of the panel in LDS
+    const int ntiles = (m + 15) / 16;
+    const double2 z = make_double2(0.0, 0.0);
+    double2 x[kLqCols];
+    for (int q = 0; q < panels; ++q) {
+        const int r0 = q * 16;
+        const bool live = active && r0 < c;
+        const int n = c - r0, nb = min(16, c - r0);
+        const bool owner = live && q % nworkers == me;
+        if (owner) {  // factorise panel q
+            const int row = q * 16 + team;
+            lq_load_row<XL>(Wg, ld, row, r0, n, m, l32, x);
+            for (int i = 0; i < nb; ++i) {
+                if (team == i) {
+                    // reflector i from row i beyond column i: row (I - tau v v^H) = (beta, 0, ...)
+                    double part = 0.0;
+#pragma unroll
+                    for (int u = 0; u < kLqCols; ++u) {
+                        const int jj = l32 + 32 * u;
+                        if (jj > i && jj < n) part += cnorm(x[u]);
+                    }
+                    const double xn2 = half_sum(part);
+                    // column i < 16 sits in slot 0 of lane i
+                    const double2 alpha = make_double2(__shfl(x[0].x, i, 32), -__shfl(x[0].y, i, 32));
+                    double2 tau, scale;
+                    double beta;
+                    if (xn2 == 0.0 && alpha.y == 0.0) {
+                        tau = z;
+                        scale = z;
+                        beta = alpha.x;
+                    } else {
+                        beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+                        const double rb = 1.0 / beta;
+                        tau = make_double2((beta - alpha.x) * rb, -alpha.y * rb);
+                        const double2 d = make_double2(alpha.x - beta, alpha.y);  // 1 / (alpha - beta)
+                        const double rdd = 1.0 / (d.x * d.x + d.y * d.y);
+                        scale = make_double2(d.x * rdd, -d.y * rdd);
+                    }
+                    double2* vg = aux + (std::size_t)(r0 + i) * ld + r0;
+                    double2* vi = vl + i * S;
+#pragma unroll
+                    for (int u = 0; u < kLqCols; ++u) {
+                        const int jj = l32 + 32 * u;
+                        if (jj >= i && jj < n) {
+                            const double2 e = jj == i ? make_double2(1.0, 0.0) : cmul(cconj(x[u]), scale);
+                            vi[jj] = e;
+                            vg[jj] = e;
+                            x[u] = jj == i ? make_double2(beta, 0.0) : z;  // row i of L
+                        }
+                    }
+                    if (l32 == 0) {
+                        tauv[i] = tau;
+                        taus[r0 + i] = tau;
+                    }
+                }
+                __syncthreads();
+                if (team > i) lq_reflect(x, vl + i * S, i, n, l32, tauv[i]);
+            }
+            lq_store_row(Wg, ld, row, r0, n, m, l32, x);
+        }
+        sync();
+        if (!live) continue;
+        bool staged = owner;  // the owner's LDS already holds the panel
+        for (int t = q + 1; t < ntiles; ++t) {
+            if (t % nworkers != me) continue;
+            if (!staged) {  // the panel's reflectors into LDS, all loads of a thread in flight
+                for (int base = tid; base < nb * n; base += 4 * (int)blockDim.x) {
+                    double2 v[4];
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const int idx = base + h * (int)blockDim.x, k = idx / n, jj = idx - k * n;
+                        v[h] = (idx < nb * n && jj >= k) ? gld<XL>(aux + (std::size_t)(r0 + k) * ld + r0 + jj) : z;
+                    }
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const int idx = base + h * (int)blockDim.x, k = idx / n, jj = idx - k * n;
+                        if (idx < nb * n) vl[k * S + jj] = v[h];
+                    }
+                }
+                if (tid < nb) tauv[tid] = gld<XL>(taus + r0 + tid);
+                __syncthreads();
+                staged = true;
+            }
+            const int row = t * 16 + team;
+            lq_load_row<XL>(Wg, ld, row, r0, n, m, l32, x);
+            for (int i = 0; i < nb; ++i) lq_reflect(x, vl + i * S, i, n, l32, tauv[i]);
+            lq_store_row(Wg, ld, row, r0, n, m, l32, x);
+        }
+        __syncthreads();  // the next panel rewrites the LDS reflectors
+    }
+    sync();
+}
+
+
+
 // How a block-Jacobi SVD is synchronised with others in the same launch. Alone:
 // every sync() is its own, rounds and sweeps follow its own matrix. Lockstep:
 // several SVDs (one per group of workgroups) share one grid barrier, so every
@@ -1088,7 +1115,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     unsigned long long tp = prof ? wall_clock64() : 0;
     if (aux) {  // W <- L of W = L Q^H (same left singular vectors and values)
         const int panels = mode.lq_panels >= 0 ? mode.lq_panels : (c + 15) / 16;
-        lq_precondition<Sync, XL>(Wg, ld, m, c, aux, taus, me, nworkers, sync, lds, sh, panels, mode.active);
+        lq_precondition<Sync, XL>(Wg, ld, m, c, aux, taus, me, nworkers, sync, lds, panels, mode.active);
     }
     if (prof) {
         const unsigned long long t = wall_clock64();
@@ -2592,7 +2619,8 @@ void Tebd::two_site(int p, const Gate4& g, const double* wts, int edge) {
         const int nbp = std::max(2, nb + (nb & 1));
         const unsigned nwg = (unsigned)(nbp / 2) * (unsigned)xcd_stride_;
         // the LDS columns of a round, or the row layout's reduction area
-        const std::size_t shm = std::max((std::size_t)2 * kBlk * mcap * sizeof(double2), sizeof(RowLds));
+        const std::size_t shm = std::max({(std::size_t)2 * kBlk * mcap * sizeof(double2), sizeof(RowLds),
+                                          lq_lds_bytes(ccap)});
         int* flags = flags_.p;
         int pp = p, cc = chi_;
         unsigned long long* stats = stats_.p;
@@ -2819,7 +2847,7 @@ static void exec_shape(int maxcap, int req, bool solo, bool group_sync, int work
     const std::size_t svd_bytes = (std::size_t)2 * kBlk * (2 * maxcap) * sizeof(double2);
     // solo: room for a kSmall x kSmall theta of svd_small
     const std::size_t small_bytes = solo ? (std::size_t)kSmall * kSmall * sizeof(double2) : 0;
-    shm = std::max({svd_bytes, small_bytes, sizeof(RowLds)});
+    shm = std::max({svd_bytes, small_bytes, sizeof(RowLds), lq_lds_bytes(cmax)});
     int per_cu = 0, cus = 0, dev = 0;
     TQ_HIP(hipGetDevice(&dev));
     TQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

@@ -30,6 +30,14 @@ FLAGS = [
 ]
 
 
+# Per-source flags. tebd.hip contracts a*b+c into an FMA only inside one source
+# expression (-ffp-contract=on), never across statements after inlining: its SVD
+# code is instantiated once per executor layout, and with cross-statement
+# contraction each instantiation could round differently, while the layouts are
+# meant to apply bit-identical rotations.
+SOURCE_FLAGS = {"tebd.hip": ["-ffp-contract=on"]}
+
+
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and Path(cand).exists():
@@ -46,7 +54,7 @@ def build(verbose: bool = False) -> Path:
     procs = []
     for src in SOURCES:
         obj = objdir / (src + ".o")
-        cmd = [cc, *FLAGS, "-I", str(CSRC), "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [cc, *FLAGS, *SOURCE_FLAGS.get(src, []), "-I", str(CSRC), "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

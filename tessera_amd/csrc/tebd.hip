@@ -982,6 +982,82 @@ __device__ void lq_precondition(double2* __restrict__ Wg, int ld, int m, int c, 
 
 
 
+// ---- convergence check on the matrix cores --------------------------------
+// After a sweep with few rotations: every column pair's coupling from one Gram
+// matrix G = W^H W (16 x 16 tiles of complex FP64 MFMA products, one tile per
+// wave at a time, K = the rows in steps of 4) against the Jacobi tolerance,
+// with the sweep's noise floor and exact column norms: the test a sweep applies
+// pair by pair, here to every pair at once. When no pair fails, the SVD has
+// converged and ends without the sweep that would only confirm it (that sweep's
+// own products could round a coupling at the tolerance to the other side; the
+// decision is the check's in every layout, so the layouts stay identical).
+// Every worker takes every (nworkers * 8)-th tile of the upper triangle; the
+// workers' results meet in *vflag (zeroed again by worker 0 after the last
+// read). compute = false joins only the barriers.
+template <class Sync, bool XL>
+__device__ bool gram_converged(const double2* __restrict__ Wg, int ld, int m, int c, double floor2,
+                               double* __restrict__ nrm, int* vflag, int me, int nworkers, Sync& sync,
+                               bool compute) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nw = nworkers * (kDistThreads / 64), gw = me * (kDistThreads / 64) + wave;
+    if (!compute) c = 0;  // only the barriers (lockstep: another group checks)
+    for (int col = gw; col < c; col += nw) {  // exact norms^2
+        const double2* x = Wg + (std::size_t)col * ld;
+        double a = 0.0;
+        for (int row = lane; row < m; row += 64) a += cnorm(gld<XL>(x + row));
+        a = wave_sum(a);
+        if (lane == 0) nrm[col] = a;
+    }
+    sync();
+    const int nt = (c + 15) / 16, ntt = nt * (nt + 1) / 2;
+    const double tol = jacobi_tol(m), tol2 = tol * tol;
+    const int lg = lane >> 4, lc = lane & 15;
+    const double2 z = make_double2(0.0, 0.0);
+    bool bad = false;
+    for (int tt = gw; tt < ntt; tt += nw) {
+        int I = 0, rem = tt;  // tile (I, J), I <= J, row-major over the upper triangle
+        while (rem >= nt - I) {
+            rem -= nt - I;
+            ++I;
+        }
+        const int J = I + rem;
+        const int ci = I * 16 + lc, cj = J * 16 + lc;  // this lane's operand columns
+        const double2* pa = Wg + (std::size_t)min(ci, c - 1) * ld;
+        const double2* pb = Wg + (std::size_t)min(cj, c - 1) * ld;
+        CTile g;
+        g.zero();
+        for (int k0 = 0; k0 < m; k0 += 32) {  // eight K steps of loads in flight
+            double2 a[8], b[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + 4 * u + lg;
+                a[u] = (k < m && ci < c) ? cconj(gld<XL>(pa + k)) : z;
+                b[u] = (k < m && cj < c) ? gld<XL>(pb + k) : z;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) g.mac(a[u], b[u]);
+        }
+        // this lane holds G[16 I + lg + 4 e][16 J + lc]
+        const int j = J * 16 + lc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int i = I * 16 + lg + 4 * e;
+            if (i < j && j < c) {
+                const double al = gld<XL>(nrm + i), be = gld<XL>(nrm + j);
+                const double2 gg = g.get(e);
+                const double g2 = gg.x * gg.x + gg.y * gg.y;
+                if (al > floor2 && be > floor2 && g2 > 0.0 && g2 > tol2 * al * be) bad = true;
+            }
+        }
+    }
+    if (__ballot(bad) && lane == 0) atomicOr(vflag, 1);
+    sync();
+    const bool ok = compute && __hip_atomic_load(vflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    sync();
+    if (me == 0 && tid == 0) *vflag = 0;
+    return ok;
+}
+
 // How a block-Jacobi SVD is synchronised with others in the same launch. Alone:
 // every sync() is its own, rounds and sweeps follow its own matrix. Lockstep:
 // several SVDs (one per group of workgroups) share one grid barrier, so every
@@ -991,7 +1067,7 @@ __device__ void lq_precondition(double2* __restrict__ Wg, int ld, int m, int c, 
 struct SvdMode {
     int rounds = -1;            // < 0: own tournament length
     int lq_panels = -1;         // LQ panels (< 0: own ceil(c/16))
-    int* gflags = nullptr;      // grid-wide rotation flags, [2]
+    int* gflags = nullptr;      // grid-wide rotation flags [2], a check asked for [1]
     bool grid_leader = false;   // clears gflags
     bool active = true;         // false: this group has no matrix
     bool prof = false;          // this workgroup records the phase profile in sh.tm
@@ -1007,6 +1083,9 @@ struct SvdMode {
     // the rotation threshold); 0: they end after a sweep that rotates nothing
     double stop_ratio = 0.0;
     bool rows = true;           // rounds in the row layout (round_rows); false: columns through LDS
+    // a sweep with few rotations is followed by a check of every pair on the matrix
+    // cores (gram_converged) that ends the SVD instead of a sweep that rotates nothing
+    bool gram = true;
     bool step_prof = false;     // with prof: cycle profile of the row-layout steps (slows them)
 };
 
@@ -1090,27 +1169,35 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     const int nbp = nb + (nb & 1) < 2 ? 2 : nb + (nb & 1);
     const int P = nbp / 2;
     const int rounds = mode.rounds >= 0 ? mode.rounds : nbp - 1;
-    double* fro = reinterpret_cast<double*>(flags + 2);  // |theta|_F^2 accumulator
     if (me == 0 && tid == 0) {
         flags[0] = 0;
         flags[1] = 0;
-        *fro = 0.0;
+        for (int q = 4; q < 8; ++q) flags[q] = 0;
     }
     if (tid == 0) {
         sh.vis = 0;
         sh.rots = 0;
     }
     sync();
+    // |theta|_F^2 for the noise floor: column norms^2 into sv_out (scratch), then
+    // summed in column order by every wave, so every worker and every layout gets
+    // the same bits
     if (mode.active)
         for (int col = me * (kDistThreads / 64) + wave; col < c; col += nworkers * (kDistThreads / 64)) {
             const double2* x = Wg + (std::size_t)col * ld;
             double a = 0.0;
             for (int row = lane; row < m; row += 64) a += cnorm(gld<XL>(x + row));
             a = wave_sum(a);
-            if (lane == 0) atomicAdd(fro, a);
+            if (lane == 0) sv_out[col] = a;
         }
     sync();
-    const double floor2 = noise_floor2(__hip_atomic_load(fro, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    double fro2 = 0.0;
+    if (mode.active) {
+        double s = 0.0;
+        for (int col = lane; col < c; col += 64) s += gld<XL>(sv_out + col);
+        fro2 = wave_sum(s);
+    }
+    const double floor2 = noise_floor2(fro2);
     const bool prof = mode.prof && tid == 0;
     unsigned long long tp = prof ? wall_clock64() : 0;
     if (aux) {  // W <- L of W = L Q^H (same left singular vectors and values)
@@ -1138,7 +1225,11 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     int ep = p2p ? __hip_atomic_load(&mode.epochs[(std::size_t)me * 64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     int sweeps_done = 0, ph = 0;
     unsigned long long nvis = 0, nrot = 0;
+    const bool gram = mode.gram && mode.rows && mode.stop_ratio == 0.0;
+    // a check pays off once a sweep rotates at most about 1/256 of the pairs
+    const int gram_rot = max(16, c * (c - 1) / 512);
     for (int sweep = 0; sweep < 40; ++sweep) {
+        int srot = 0;  // thread 0: this worker's rotations in the sweep
         if (working) ++sweeps_done;
         if (prof && working) sh.tm[kProfHist + 2 * kHistSweeps + min(sweep, kHistSweeps - 1)] += 1;
         for (int r = 0; r < rounds; ++r) {
@@ -1208,6 +1299,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                         nvis += (unsigned long long)nv;
                         nrot += (unsigned long long)nr;
                         wrot = nr;
+                        srot += nr;
                     }
                 } else {
                 double2 xa[kRowsPerLane], ya[kRowsPerLane];
@@ -1328,6 +1420,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 }
             } else {
                 const unsigned long long tw = prof ? wall_clock64() : 0;
+                if (gram && r == rounds - 1 && tid == 0 && srot) atomicAdd(&flags[4 + (sweep & 1)], srot);
                 sync();
                 if (prof) {
                     sh.tm[8] += wall_clock64() - tw;
@@ -1341,13 +1434,35 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
         const int gf = mode.gflags ? __hip_atomic_load(&mode.gflags[sweep & 1], __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_AGENT)
                                    : f;
+        const int cnt = __hip_atomic_load(&flags[4 + (sweep & 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // this SVD asks for the convergence check; in lockstep every group joins the
+        // check's grid barriers when any group asks (gflags[2])
+        const bool want = gram && working && (f & 2) && cnt <= gram_rot;
+        if (want && mode.gflags && me == 0 && tid == 0) atomicOr(&mode.gflags[2], 1);
         sync();
-        if (me == 0 && tid == 0) flags[sweep & 1] = 0;
+        if (me == 0 && tid == 0) {
+            flags[sweep & 1] = 0;
+            flags[4 + (sweep & 1)] = 0;
+        }
         if (mode.grid_leader && tid == 0 && mode.gflags) mode.gflags[sweep & 1] = 0;
         if (prof) sh.tm[kProfRound + 1] += wall_clock64() - tend;
         // bit 1: a rotation above stop_ratio (every rotation when stop_ratio = 0)
         if (!(f & 2)) working = false;
         if (!(gf & 2)) break;
+        const bool run = mode.gflags
+                             ? __hip_atomic_load(&mode.gflags[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0
+                             : want;
+        if (run) {
+            const unsigned long long tg = prof ? wall_clock64() : 0;
+            const bool done =
+                gram_converged<Sync, XL>(Wg, ld, m, c, floor2, sv_out, flags + 6, me, nworkers, sync, want);
+            if (mode.grid_leader && tid == 0 && mode.gflags) mode.gflags[2] = 0;
+            if (prof) sh.tm[kProfRound + 1] += wall_clock64() - tg;
+            if (want && done) {
+                working = false;
+                if (!mode.gflags) break;
+            }
+        }
     }
     if (prof) {
         sh.tm[2] += wall_clock64() - tp;
@@ -2037,8 +2152,8 @@ struct ExecArgs {
     int ld;
     double* sv;
     int* perm;
-    int* svd_flags;     // [G][4]: rotation flags and a double accumulator per group
-    int* gflags;        // [2]
+    int* svd_flags;     // [G][8]: rotation flags [0, 1], rotation counts [4, 5], check flag [6]
+    int* gflags;        // [3]: grid-wide rotation flags (per sweep parity), a convergence check asked for
     double2* snap;
     std::size_t snap_stride;
     int* snap_dims;     // [slots][n + 1]
@@ -2051,6 +2166,7 @@ struct ExecArgs {
     int naps;              // 64-cycle sleeps between the polls of a wait
     double stop_ratio;     // SvdMode::stop_ratio of every SVD
     int rows;              // SvdMode::rows of every SVD
+    int gram;              // SvdMode::gram of the non-lockstep SVDs
     int step_prof;         // SvdMode::step_prof of the profiled workgroup
     int prof_on;           // workgroup 0 records the phase profile
     int* epochs;        // [G][W][64] round counters of the point-to-point rounds (per-group layout)
@@ -2150,13 +2266,14 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
     double2* W = a.work + grp * mat_sz;
     double* sv = a.sv + (std::size_t)grp * a.ld;
     int* perm = a.perm + (std::size_t)grp * a.ld;
-    int* sflags = a.svd_flags + 4 * grp;
+    int* sflags = a.svd_flags + 8 * grp;
     const long long gid = (long long)rank * blockDim.x + tid, gstride = (long long)nwg * blockDim.x;
     const long long tid_all = (long long)blockIdx.x * blockDim.x + tid;
     const long long all = (long long)gridDim.x * blockDim.x;
     if (blockIdx.x == 0 && tid == 0) {
         a.gflags[0] = 0;
         a.gflags[1] = 0;
+        a.gflags[2] = 0;
     }
     if constexpr (XL) {
         // every group must sit on one XCD (blockIdx mod 8 = XCC id, G a multiple of
@@ -2227,6 +2344,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                     SvdMode mode;  // own tournament length, own convergence
                     mode.stop_ratio = a.stop_ratio;
                     mode.rows = a.rows != 0;
+                    mode.gram = a.gram != 0;
                     svd_blocks(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, 0, 1, bs, lds,
                                sh, mode, (a.aux ? a.aux + grp * mat_sz : nullptr), a.taus + (std::size_t)grp * a.ld);
                 }
@@ -2258,6 +2376,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 mode.prof = a.prof_on && blockIdx.x == 0;
                 mode.stop_ratio = a.stop_ratio;
                 mode.rows = a.rows != 0;
+                mode.gram = a.gram != 0;
                 mode.step_prof = a.step_prof != 0;
                 mode.epochs = a.epochs ? a.epochs + (std::size_t)grp * nwg * 64 : nullptr;
                 mode.err = a.err;
@@ -2295,6 +2414,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
             mode.step_prof = a.step_prof != 0;
             mode.rounds = a.rounds[st];
             mode.gflags = a.gflags;
+            mode.gram = a.gram != 0;
             mode.grid_leader = blockIdx.x == 0;
             mode.active = active;
             mode.lq_panels = (a.ld + 15) / 16;
@@ -2375,7 +2495,7 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     maxcap_ = maxcap;
     const int ld = 2 * maxcap;
     ld_ = ld;
-    flags_.alloc(4);  // two rotation flags + a double accumulator
+    flags_.alloc(8);  // rotation flags [0, 1], rotation counts [4, 5], check flag [6]
     stats_.alloc(4 * (std::size_t)kStatSlots);
     TQ_HIP(hipMemset(stats_.p, 0, 4 * (std::size_t)kStatSlots * sizeof(unsigned long long)));
     mat_.alloc((std::size_t)ld * ld);
@@ -2434,6 +2554,10 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     // instead of the row layout
     const char* rw = std::getenv("TESSERA_SVD_ROWS");
     svd_rows_ = !(rw && rw[0] == '0');
+    // knob: 0 keeps the executor's SVDs from ending with the matrix-core convergence
+    // check (they then end after a sweep that rotates nothing)
+    const char* gm = std::getenv("TESSERA_SVD_GRAM");
+    svd_gram_ = !(gm && gm[0] == '0');
     // knob: 1 adds the cycle profile of the row-layout rotation steps to the
     // executor profile (its timer reads slow the profiled workgroup)
     const char* sp = std::getenv("TESSERA_EXEC_STEP_PROFILE");
@@ -3088,7 +3212,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
         t0.ex_perm_.alloc((std::size_t)G * ld);
         t0.ex_taus_.alloc((std::size_t)G * ld);
     }
-    if (t0.ex_flags_.n < 4 * (std::size_t)G + 2) t0.ex_flags_.alloc(4 * (std::size_t)G + 2);
+    if (t0.ex_flags_.n < 8 * (std::size_t)G + 3) t0.ex_flags_.alloc(8 * (std::size_t)G + 3);
     if (t0.ex_bar_.n < 2 * (std::size_t)G + 1) t0.ex_bar_.alloc(2 * (std::size_t)G + 1);
     if (!t0.ex_prof_.p) {
         t0.ex_prof_.alloc(kProfSlots);
@@ -3148,7 +3272,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.sv = t0.ex_sv_.p;
     x.perm = t0.ex_perm_.p;
     x.svd_flags = t0.ex_flags_.p;
-    x.gflags = t0.ex_flags_.p + 4 * G;
+    x.gflags = t0.ex_flags_.p + 8 * G;
     x.snap = t0.ex_snap_.p;
     x.snap_stride = t0.off_[t0.n_];
     x.snap_dims = t0.ex_sdims_.p;
@@ -3159,6 +3283,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.naps = t0.ex_naps_;
     x.stop_ratio = t0.svd_stop_;
     x.rows = t0.svd_rows_ ? 1 : 0;
+    x.gram = t0.svd_gram_ ? 1 : 0;
     x.step_prof = t0.ex_step_prof_ ? 1 : 0;
     x.prof_on = t0.ex_prof_on_ ? 1 : 0;
     x.dataflow = (group_sync && t0.ex_dataflow_ && snap_list.empty()) ? 1 : 0;

@@ -197,6 +197,7 @@ private:
     int ex_naps_ = 1;                      // 64-cycle sleeps between the polls of a wait
     double svd_stop_ = 0.0;                // SvdMode::stop_ratio of the executor's SVDs
     bool svd_rows_ = true;                 // SvdMode::rows of the executor's SVDs
+    bool svd_gram_ = true;                 // SvdMode::gram of the executor's SVDs
     bool ex_step_prof_ = false;            // cycle profile of the row-layout steps
     bool ex_prof_on_ = true;               // workgroup 0's phase profile
     bool ex_p2p_ = true;                   // per-group layout: point-to-point tournament rounds

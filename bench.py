@@ -74,7 +74,8 @@ def parse():
     ap.add_argument("--no-lq", action="store_true",
                     help="tebd: Jacobi directly on theta, without the LQ preconditioning of the block SVD")
     ap.add_argument("--step-profile", action="store_true",
-                    help="tebd: add the executor's cycle profile of the Jacobi rotation steps (slows workgroup 0)")
+                    help="tebd: add the executor's cycle profile of the Jacobi rotation steps (slows workgroup 0; "
+                         "zeros unless the library was built with TESSERA_STEP_PROFILE=1)")
     ap.add_argument("--exec-workers", type=int, default=0,
                     help="lockstep executor: workgroups per group (0: one per block pair of a round)")
     ap.add_argument("--instances", type=int, default=32, help="batch workload: number of batched-workload instances")

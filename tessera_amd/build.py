@@ -36,6 +36,10 @@ FLAGS = [
 # contraction each instantiation could round differently, while the layouts are
 # meant to apply bit-identical rotations.
 SOURCE_FLAGS = {"tebd.hip": ["-ffp-contract=on"]}
+# TESSERA_STEP_PROFILE=1 in the environment builds the executor's cycle profile of
+# the Jacobi rotation steps (it costs the steps registers, so it is off by default)
+if os.environ.get("TESSERA_STEP_PROFILE") == "1":
+    SOURCE_FLAGS["tebd.hip"] = SOURCE_FLAGS["tebd.hip"] + ["-DTESSERA_STEP_PROFILE=1"]
 
 
 def hipcc() -> str:

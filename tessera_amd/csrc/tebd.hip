@@ -469,7 +469,7 @@ constexpr int kHistSweeps = 24;  // sweep histogram: sweeps past the last fold i
 constexpr int kProfHist = 17;    // first histogram slot
 constexpr int kProfStep = kProfHist + 3 * kHistSweeps;  // row-layout step parts (cycles)
 constexpr int kProfRound = kProfStep + 5;  // whole rounds, sweep ends (wall ticks)
-constexpr int kProfSlots = kProfRound + 2;
+constexpr int kProfSlots = kProfRound + 4;
 
 struct SvdShared {
     unsigned long long tm[kProfSlots];  // phase profile (workgroup 0, thread 0)
@@ -660,6 +660,13 @@ __device__ __forceinline__ void row_norms(const double2 (&v)[16], RowLds& rl, in
     wave_lds_fence();
 }
 
+// The cycle profile of the row-layout steps costs registers in the step's hot
+// path, so it exists only in builds with -DTESSERA_STEP_PROFILE=1.
+#ifndef TESSERA_STEP_PROFILE
+#define TESSERA_STEP_PROFILE 0
+#endif
+constexpr bool kStepProfile = TESSERA_STEP_PROFILE != 0;
+
 // One step of 8 pair rotations (KIND, S as in row_pair_a). cvalid: bit q set
 // when column q exists. Accumulates visited / rotated pairs and the rotation
 // flags (bit 0 any, bit 1 a coupling above big_tol).
@@ -667,10 +674,11 @@ template <int KIND, int S>
 __device__ __forceinline__ void row_step(double2 (&v)[16], RowLds& rl, int& ph, int lane, int wave, unsigned cvalid, int m,
                                 double floor2, double big_tol, int& flags, int& nvis, int& nrot,
                                 unsigned long long* tm) {
-    // tm (thread 0 of the profiled workgroup): cycles of the step's parts
-    unsigned long long tc = tm ? __builtin_readcyclecounter() : 0;
+    // tm (thread 0 of the profiled workgroup, in builds with TESSERA_STEP_PROFILE
+    // set): cycles of the step's parts
+    unsigned long long tc = (kStepProfile && tm) ? __builtin_readcyclecounter() : 0;
     auto mark = [&](int q) {
-        if (tm) {
+        if (kStepProfile && tm) {
             const unsigned long long t = __builtin_readcyclecounter();
             tm[q] += t - tc;
             tc = t;
@@ -700,22 +708,28 @@ __device__ __forceinline__ void row_step(double2 (&v)[16], RowLds& rl, int& ph, 
     bool rot = false, big = false, need = false;
     if (vis && al > floor2 && be > floor2) {
         // the rotation of pair_rotate_carried with squared tests and refined
-        // hardware reciprocals in place of divides and square roots: the phase
-        // and cs keep unit modulus and cs^2 (1 + t^2) = 1 to rounding, so the
-        // applied transformation stays unitary
+        // hardware reciprocal square roots in place of divides and square roots.
+        // With d = beta - alpha and zeta = d / (2 |gamma|), the rotation with
+        // t = sgn(zeta) / (|zeta| + sqrt(1 + zeta^2)) has
+        //   cs^2 = u = (1 + |d| h) / 2,  sn = sgn(d) |gamma| h / sqrt(u),
+        //   t |gamma| = sgn(d) |gamma|^2 h / u,  h = 1 / sqrt(4 |gamma|^2 + d^2),
+        // so two independent reciprocal square roots (of |gamma|^2 for the phase,
+        // and of 4 |gamma|^2 + d^2) and one of u form it; cs^2 + sn^2 = 1 and the
+        // phase has unit modulus to rounding, so the transformation stays unitary
         const double g2 = gr * gr + gi * gi, ab = al * be, tol = jacobi_tol(m);
         if (g2 > 0.0 && g2 > tol * tol * ab) {
             rot = true;
             big = g2 > big_tol * big_tol * ab;
-            const double rg = rsq_nr(g2), gabs = g2 * rg;
+            const double d = be - al;
+            const double rg = rsq_nr(g2), h = rsq_nr(fma(4.0, g2, d * d));
             phx = gr * rg;  // conj(gamma)/|gamma|
             phy = -gi * rg;
-            const double zeta = 0.5 * (be - al) * rg;
-            const double w = 1.0 + zeta * zeta;
-            const double t = (zeta >= 0 ? 1.0 : -1.0) * rcp_nr(fabs(zeta) + w * rsq_nr(w));
-            cs = rsq_nr(1.0 + t * t);
-            sn = cs * t;
-            const double al1 = al - t * gabs, be1 = be + t * gabs;
+            const double u = fma(0.5 * fabs(d), h, 0.5), q = rsq_nr(u);
+            const double sg = d >= 0 ? 1.0 : -1.0;
+            cs = u * q;
+            sn = sg * (g2 * rg) * h * q;
+            const double tg = sg * g2 * h * (q * q);  // t |gamma|
+            const double al1 = al - tg, be1 = be + tg;
             need = al1 < al * (1.0 / 64) || be1 < be * (1.0 / 64);
             al = al1;
             be = be1;
@@ -1132,20 +1146,26 @@ __device__ inline int tour_holder(int r, int blk, int P) {
 // Thread 0 of a workgroup waits until a round counter reaches `target`; every
 // wait ends (counter reached, another wait gave up, or the poll limit, which
 // raises *err).
-// Between polls the thread sleeps `naps` times 64 cycles: each device-scope
-// acquire poll invalidates the XCD's L2 lines that other workgroups are reading.
+// Between polls the thread sleeps `naps` times 64 cycles. The polls are relaxed
+// (a device-scope acquire would invalidate the XCD's L2 lines that other
+// workgroups are reading at every poll); one acquire fence follows the poll that
+// saw the target, which synchronises with the producer's release store that the
+// poll read. The error word is read every 32nd poll only, so a poll costs one
+// memory round trip.
 template <bool XL = false>
 __device__ inline void wait_counter(int* ctr, int target, int* err, long long limit, int naps = 1) {
-    // XCD-local: the poll reads the L2 (sc1) and acquires nothing; the caller
-    // reads the producer's data with L1-bypassing loads
-    constexpr int order = XL ? __ATOMIC_RELAXED : __ATOMIC_ACQUIRE;
-    for (long long spins = 0; __hip_atomic_load(ctr, order, __HIP_MEMORY_SCOPE_AGENT) < target; ++spins) {
-        if (spins >= limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    for (long long spins = 0; __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target;
+         ++spins) {
+        if (spins >= limit ||
+            ((spins & 31) == 31 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
             __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return;
         }
         for (int q = 0; q < naps; ++q) __builtin_amdgcn_s_sleep(1);
     }
+    // XCD-local: nothing to acquire; the caller reads the producer's data with
+    // L1-bypassing loads
+    if constexpr (!XL) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // Block-Jacobi SVD run by `nworkers` workgroups that synchronise with `sync()`.
@@ -1229,11 +1249,13 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     // a check pays off once a sweep rotates at most about 1/256 of the pairs
     const int gram_rot = max(16, c * (c - 1) / 512);
     for (int sweep = 0; sweep < 40; ++sweep) {
-        int srot = 0;  // thread 0: this worker's rotations in the sweep
+        int srot = 0;    // thread 0: this worker's rotations in the sweep
+        int sflags = 0;  // thread 0: its rotation flags not yet published
         if (working) ++sweeps_done;
         if (prof && working) sh.tm[kProfHist + 2 * kHistSweeps + min(sweep, kHistSweeps - 1)] += 1;
         for (int r = 0; r < rounds; ++r) {
             const unsigned long long tround = prof ? wall_clock64() : 0;
+            unsigned long long tpost = tround;
             if (p2p && r > 0) {
                 const unsigned long long tw = prof ? wall_clock64() : 0;
                 if (tid == 0) {
@@ -1247,10 +1269,10 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                         pa = rr_holder(r - 1, bi, nbp);
                         pb = rr_holder(r - 1, bj, nbp);
                     }
-                    // device scope: the polls are acquire loads, so the producers'
-                    // blocks (behind their release) are visible to the whole workgroup
-                    // after the barrier below; XCD-local: the blocks are read with
-                    // L1-bypassing loads
+                    // device scope: each wait ends with an acquire fence, so the
+                    // producers' blocks (behind their release) are visible to the whole
+                    // workgroup after the barrier below; XCD-local: the blocks are read
+                    // with L1-bypassing loads
                     if (pa != me)
                         wait_counter<XL>(&mode.epochs[(std::size_t)pa * 64], ep, mode.err, mode.spin_limit, mode.naps);
                     if (pb != me)
@@ -1281,6 +1303,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 }
                 if (tid == 0) sh.rot = 0;
                 unsigned long long tr0 = prof ? wall_clock64() : 0;
+                if (prof) sh.tm[kProfRound + 2] += tr0 - tround;
                 auto lap = [&](int slot) {  // round phases of workgroup 0
                     if (prof) {
                         const unsigned long long t = wall_clock64();
@@ -1289,12 +1312,14 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                     }
                 };
                 int rflags = 0, wrot = 0;
+                tpost = 0;
                 if (mode.rows) {
                     int nv = 0, nr = 0;
                     rflags = round_rows<XL>(Wg, ld, m, c, bi, bj, r == 0, floor2, mode.stop_ratio,
                                             *reinterpret_cast<RowLds*>(lds), ph, nv, nr,
                                             (prof && mode.step_prof) ? sh.tm + kProfStep : nullptr, rv, load_a,
                                             store_a, lap);
+                    tpost = prof ? wall_clock64() : 0;
                     if (tid == 0) {  // every wave holds the same counts
                         nvis += (unsigned long long)nv;
                         nrot += (unsigned long long)nr;
@@ -1387,15 +1412,15 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 }
                 if (mode.prof && lane == 0 && wrot)
                     atomicAdd(&sh.tm[kProfHist + min(sweep, kHistSweeps - 1)], (unsigned long long)wrot);
-                if (tid == 0 && rflags) {
-                    atomicOr(&flags[sweep & 1], rflags);
-                    if (mode.gflags) atomicOr(&mode.gflags[sweep & 1], rflags);
+                if (tid == 0) {
+                    sflags |= rflags;
                     if (mode.prof && (rflags & 2))
                         sh.tm[kProfHist + kHistSweeps + min(sweep, kHistSweeps - 1)] += 1;
                 }
             }
             if (p2p && r < rounds - 1) {
                 const unsigned long long tpub = prof ? wall_clock64() : 0;
+                if (prof) sh.tm[kProfRound + 3] += tpub - tpost;
                 // publish this round's blocks: every wave waits until its write-back
                 // stores are complete in the L2 (s_waitcnt of all counters; stores
                 // count in vmcnt on CDNA), the workgroup barrier follows, and thread 0's
@@ -1420,6 +1445,12 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 }
             } else {
                 const unsigned long long tw = prof ? wall_clock64() : 0;
+                // the sweep's rotation flags, read only after its last barrier
+                if (tid == 0 && sflags) {
+                    atomicOr(&flags[sweep & 1], sflags);
+                    if (mode.gflags) atomicOr(&mode.gflags[sweep & 1], sflags);
+                    sflags = 0;
+                }
                 if (gram && r == rounds - 1 && tid == 0 && srot) atomicAdd(&flags[4 + (sweep & 1)], srot);
                 sync();
                 if (prof) {
@@ -2362,7 +2393,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                     if (tid == 0) {
                         for (int q = 0; q < 2; ++q)
                             if (u.pred[q] >= 0) wait_counter(&a.done[u.pred[q]], 1, a.err, a.spin_limit);
-                        // the polls are acquire loads: the predecessors' sites are visible
+                        // each wait ends with an acquire fence: the predecessors' sites are visible
                     }
                     __syncthreads();
                     mark(6);

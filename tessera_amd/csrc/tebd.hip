@@ -547,13 +547,14 @@ struct RowLds {
     double par[kDistThreads / 64][8][4];   // each wave's rotations: cs, sn, conj(g)/|g|
 };
 
-// Total over the waves of partial value (lane & 15) of buffer p: 8 contiguous
-// doubles per lane, read together.
+// Total over the NW waves of partial value (lane & 15) of buffer p: NW
+// contiguous doubles per lane, read together.
+template <int NW>
 __device__ __forceinline__ double row_total(const RowLds& rl, int p, int lane) {
     const double* r = rl.red[p][lane & 15];
     double t = 0.0;
 #pragma unroll
-    for (int w = 0; w < kDistThreads / 64; ++w) t += r[w];
+    for (int w = 0; w < NW; ++w) t += r[w];
     return t;
 }
 
@@ -646,16 +647,22 @@ __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Exact norms^2 of the 16 columns into every wave's rn (one reduction).
-__device__ __forceinline__ void row_norms(const double2 (&v)[16], RowLds& rl, int& ph, int lane, int wave) {
+// Exact norms^2 of the 16 columns into every wave's rn (one reduction). RPT:
+// rows per thread (v[r] holds row tid + r * blockDim.x), so 8 / RPT waves.
+template <int RPT>
+__device__ __forceinline__ void row_norms(const double2 (&v)[RPT][16], RowLds& rl, int& ph, int lane, int wave) {
     double d[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) d[q] = cnorm(v[q]);
+    for (int q = 0; q < 16; ++q) {
+        d[q] = cnorm(v[0][q]);
+#pragma unroll
+        for (int r = 1; r < RPT; ++r) d[q] += cnorm(v[r][q]);
+    }
     const double h = wave_reduce16(d, lane);
     const int p = ph++ & 1;
     if ((lane & 3) == 0) rl.red[p][lane >> 2][wave] = h;
     __syncthreads();
-    const double t = row_total(rl, p, lane);
+    const double t = row_total<8 / RPT>(rl, p, lane);
     if (lane < 16) rl.rn[wave][lane] = t;
     wave_lds_fence();
 }
@@ -670,9 +677,9 @@ constexpr bool kStepProfile = TESSERA_STEP_PROFILE != 0;
 // One step of 8 pair rotations (KIND, S as in row_pair_a). cvalid: bit q set
 // when column q exists. Accumulates visited / rotated pairs and the rotation
 // flags (bit 0 any, bit 1 a coupling above big_tol).
-template <int KIND, int S>
-__device__ __forceinline__ void row_step(double2 (&v)[16], RowLds& rl, int& ph, int lane, int wave, unsigned cvalid, int m,
-                                double floor2, double big_tol, int& flags, int& nvis, int& nrot,
+template <int KIND, int S, int RPT>
+__device__ __forceinline__ void row_step(double2 (&v)[RPT][16], RowLds& rl, int& ph, int lane, int wave, unsigned cvalid,
+                                int m, double floor2, double big_tol, int& flags, int& nvis, int& nrot,
                                 unsigned long long* tm) {
     // tm (thread 0 of the profiled workgroup, in builds with TESSERA_STEP_PROFILE
     // set): cycles of the step's parts
@@ -687,9 +694,16 @@ __device__ __forceinline__ void row_step(double2 (&v)[16], RowLds& rl, int& ph, 
     double d[16];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const double2 x = v[row_pair_a(KIND, S, k)], y = v[row_pair_b(KIND, S, k)];
+        const double2 x = v[0][row_pair_a(KIND, S, k)], y = v[0][row_pair_b(KIND, S, k)];
         d[2 * k] = x.x * y.x + x.y * y.y;  // conj(x) y
         d[2 * k + 1] = x.x * y.y - x.y * y.x;
+#pragma unroll
+        for (int r = 1; r < RPT; ++r) {  // the thread's further rows
+            const double2 xr = v[r][row_pair_a(KIND, S, k)], yr = v[r][row_pair_b(KIND, S, k)];
+            const double re = xr.x * yr.x + xr.y * yr.y, im = xr.x * yr.y - xr.y * yr.x;
+            d[2 * k] += re;
+            d[2 * k + 1] += im;
+        }
     }
     const double h = wave_reduce16(d, lane);
     const int p = ph++ & 1;
@@ -701,7 +715,7 @@ __device__ __forceinline__ void row_step(double2 (&v)[16], RowLds& rl, int& ph, 
     // and takes Im from its odd neighbour (every 16 lanes compute the same 8
     // rotations; only the even lanes' results are used)
     const int k = (lane >> 1) & 7, a = row_pair_a(KIND, S, k), b = row_pair_b(KIND, S, k);
-    const double gr = row_total(rl, p, lane), gi = dpp_f64<0xB1>(gr);
+    const double gr = row_total<8 / RPT>(rl, p, lane), gi = dpp_f64<0xB1>(gr);
     const bool vis = ((cvalid >> a) & 1) && ((cvalid >> b) & 1);
     double al = rl.rn[wave][a], be = rl.rn[wave][b];
     double cs = 1.0, sn = 0.0, phx = 1.0, phy = 0.0;
@@ -759,25 +773,28 @@ __device__ __forceinline__ void row_step(double2 (&v)[16], RowLds& rl, int& ph, 
         for (int q = 0; q < 8; ++q) {
             const double c0 = rl.par[wave][q][0], s0 = rl.par[wave][q][1];
             const double2 e = make_double2(rl.par[wave][q][2], rl.par[wave][q][3]);
-            const double2 x = v[row_pair_a(KIND, S, q)], y = cmul(v[row_pair_b(KIND, S, q)], e);
-            v[row_pair_a(KIND, S, q)] = make_double2(c0 * x.x - s0 * y.x, c0 * x.y - s0 * y.y);
-            v[row_pair_b(KIND, S, q)] = make_double2(s0 * x.x + c0 * y.x, s0 * x.y + c0 * y.y);
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const double2 x = v[r][row_pair_a(KIND, S, q)], y = cmul(v[r][row_pair_b(KIND, S, q)], e);
+                v[r][row_pair_a(KIND, S, q)] = make_double2(c0 * x.x - s0 * y.x, c0 * x.y - s0 * y.y);
+                v[r][row_pair_b(KIND, S, q)] = make_double2(s0 * x.x + c0 * y.x, s0 * x.y + c0 * y.y);
+            }
         }
     }
     mark(4);
     nvis += __popcll(vism);
     nrot += __popcll(rotm);
     flags |= (rotm ? 1 : 0) | (bigm ? 2 : 0);
-    if (needm) row_norms(v, rl, ph, lane, wave);  // identical in every wave: a uniform branch
+    if (needm) row_norms<RPT>(v, rl, ph, lane, wave);  // identical in every wave: a uniform branch
 }
 
-template <int KIND, int S, int N>
-__device__ __forceinline__ void row_steps(double2 (&v)[16], RowLds& rl, int& ph, int lane, int wave, unsigned cvalid, int m,
-                                 double floor2, double big_tol, int& flags, int& nvis, int& nrot,
-                                 unsigned long long* tm) {
+template <int KIND, int S, int N, int RPT>
+__device__ __forceinline__ void row_steps(double2 (&v)[RPT][16], RowLds& rl, int& ph, int lane, int wave,
+                                 unsigned cvalid, int m, double floor2, double big_tol, int& flags, int& nvis,
+                                 int& nrot, unsigned long long* tm) {
     if constexpr (S < N) {
-        row_step<KIND, S>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
-        row_steps<KIND, S + 1, N>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
+        row_step<KIND, S, RPT>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
+        row_steps<KIND, S + 1, N, RPT>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
     }
 }
 
@@ -789,11 +806,13 @@ __device__ __forceinline__ void row_steps(double2 (&v)[16], RowLds& rl, int& ph,
 // 15 write-back). ph counts the LDS reductions of the workgroup: consecutive
 // reductions alternate between the two partial buffers, so one barrier each
 // suffices (a wave can be at most one reduction ahead of another).
-template <bool XL, class Lap>
+template <bool XL, int RPT, class Lap>
 __device__ __forceinline__ int round_rows(double2* __restrict__ Wg, int ld, int m, int c, int bi, int bj, bool inblock,
                                  double floor2, double big_tol, RowLds& rl, int& ph, int& nvis, int& nrot,
-                                 unsigned long long* tm, double2 (&v)[16], bool load_a, bool store_a, Lap&& lap) {
+                                 unsigned long long* tm, double2 (&v)[RPT][16], bool load_a, bool store_a,
+                                 Lap&& lap) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int kThreads = kDistThreads / RPT;  // row tid + r * kThreads in v[r]
     unsigned cvalid = 0;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -801,20 +820,28 @@ __device__ __forceinline__ int round_rows(double2* __restrict__ Wg, int ld, int 
         const bool ok = col < c;
         cvalid |= ok ? 1u << q : 0u;
         if (q >= kBlk || load_a)
-            v[q] = (ok && tid < m) ? gld<XL>(Wg + (std::size_t)col * ld + tid) : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int row = tid + r * kThreads;
+                v[r][q] = (ok && row < m) ? gld<XL>(Wg + (std::size_t)col * ld + row) : make_double2(0.0, 0.0);
+            }
     }
     int flags = 0;
-    row_norms(v, rl, ph, lane, wave);
+    row_norms<RPT>(v, rl, ph, lane, wave);
     lap(12);
     if (inblock)
-        row_steps<0, 0, kBlk - 1>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
+        row_steps<0, 0, kBlk - 1, RPT>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
     lap(13);
-    row_steps<1, 0, kBlk>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
+    row_steps<1, 0, kBlk, RPT>(v, rl, ph, lane, wave, cvalid, m, floor2, big_tol, flags, nvis, nrot, tm);
     lap(14);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int col = (q < kBlk ? bi : bj) * kBlk + (q & (kBlk - 1));
-        if ((q >= kBlk || store_a) && ((cvalid >> q) & 1) && tid < m) Wg[(std::size_t)col * ld + tid] = v[q];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int row = tid + r * kThreads;
+            if ((q >= kBlk || store_a) && ((cvalid >> q) & 1) && row < m) Wg[(std::size_t)col * ld + row] = v[r][q];
+        }
     }
     lap(15);
     return flags;
@@ -894,72 +921,81 @@ __device__ __forceinline__ void lq_reflect(double2 (&x)[kLqCols], const double2*
     }
 }
 
-template <class Sync, bool XL>
+template <class Sync, bool XL, int RPT = 1>
 __device__ void lq_precondition(double2* __restrict__ Wg, int ld, int m, int c, double2* __restrict__ aux,
                                 double2* __restrict__ taus, int me, int nworkers, Sync& sync, double2* lds,
                                 int panels, bool active) {
+    // RPT tile rows per team: 16 / RPT teams of 32 lanes (blockDim.x = 512 / RPT);
+    // tile row i belongs to team i % kTeams, register slot i / kTeams
+    constexpr int kTeams = 16 / RPT;
     const int tid = threadIdx.x, team = tid >> 5, l32 = tid & 31;
     const int S = c + 1;
     double2* vl = lds;
     double2* tauv = lds + (std::size_t)16 * S;  // taus of the panel in LDS
     const int ntiles = (m + 15) / 16;
     const double2 z = make_double2(0.0, 0.0);
-    double2 x[kLqCols];
+    double2 x[RPT][kLqCols];
     for (int q = 0; q < panels; ++q) {
         const int r0 = q * 16;
         const bool live = active && r0 < c;
         const int n = c - r0, nb = min(16, c - r0);
         const bool owner = live && q % nworkers == me;
         if (owner) {  // factorise panel q
-            const int row = q * 16 + team;
-            lq_load_row<XL>(Wg, ld, row, r0, n, m, l32, x);
-            for (int i = 0; i < nb; ++i) {
-                if (team == i) {
-                    // reflector i from row i beyond column i: row (I - tau v v^H) = (beta, 0, ...)
-                    double part = 0.0;
 #pragma unroll
-                    for (int u = 0; u < kLqCols; ++u) {
-                        const int jj = l32 + 32 * u;
-                        if (jj > i && jj < n) part += cnorm(x[u]);
-                    }
-                    const double xn2 = half_sum(part);
-                    // column i < 16 sits in slot 0 of lane i
-                    const double2 alpha = make_double2(__shfl(x[0].x, i, 32), -__shfl(x[0].y, i, 32));
-                    double2 tau, scale;
-                    double beta;
-                    if (xn2 == 0.0 && alpha.y == 0.0) {
-                        tau = z;
-                        scale = z;
-                        beta = alpha.x;
-                    } else {
-                        beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
-                        const double rb = 1.0 / beta;
-                        tau = make_double2((beta - alpha.x) * rb, -alpha.y * rb);
-                        const double2 d = make_double2(alpha.x - beta, alpha.y);  // 1 / (alpha - beta)
-                        const double rdd = 1.0 / (d.x * d.x + d.y * d.y);
-                        scale = make_double2(d.x * rdd, -d.y * rdd);
-                    }
-                    double2* vg = aux + (std::size_t)(r0 + i) * ld + r0;
-                    double2* vi = vl + i * S;
+            for (int sl = 0; sl < RPT; ++sl) lq_load_row<XL>(Wg, ld, q * 16 + team + kTeams * sl, r0, n, m, l32, x[sl]);
 #pragma unroll
-                    for (int u = 0; u < kLqCols; ++u) {
-                        const int jj = l32 + 32 * u;
-                        if (jj >= i && jj < n) {
-                            const double2 e = jj == i ? make_double2(1.0, 0.0) : cmul(cconj(x[u]), scale);
-                            vi[jj] = e;
-                            vg[jj] = e;
-                            x[u] = jj == i ? make_double2(beta, 0.0) : z;  // row i of L
+            for (int sl = 0; sl < RPT; ++sl)
+                for (int i = sl * kTeams; i < min(nb, (sl + 1) * kTeams); ++i) {
+                    if (team == i - sl * kTeams) {
+                        // reflector i from row i beyond column i: row (I - tau v v^H) = (beta, 0, ...)
+                        double2 (&xr)[kLqCols] = x[sl];
+                        double part = 0.0;
+#pragma unroll
+                        for (int u = 0; u < kLqCols; ++u) {
+                            const int jj = l32 + 32 * u;
+                            if (jj > i && jj < n) part += cnorm(xr[u]);
+                        }
+                        const double xn2 = half_sum(part);
+                        // column i < 16 sits in slot 0 of lane i
+                        const double2 alpha = make_double2(__shfl(xr[0].x, i, 32), -__shfl(xr[0].y, i, 32));
+                        double2 tau, scale;
+                        double beta;
+                        if (xn2 == 0.0 && alpha.y == 0.0) {
+                            tau = z;
+                            scale = z;
+                            beta = alpha.x;
+                        } else {
+                            beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+                            const double rb = 1.0 / beta;
+                            tau = make_double2((beta - alpha.x) * rb, -alpha.y * rb);
+                            const double2 d = make_double2(alpha.x - beta, alpha.y);  // 1 / (alpha - beta)
+                            const double rdd = 1.0 / (d.x * d.x + d.y * d.y);
+                            scale = make_double2(d.x * rdd, -d.y * rdd);
+                        }
+                        double2* vg = aux + (std::size_t)(r0 + i) * ld + r0;
+                        double2* vi = vl + i * S;
+#pragma unroll
+                        for (int u = 0; u < kLqCols; ++u) {
+                            const int jj = l32 + 32 * u;
+                            if (jj >= i && jj < n) {
+                                const double2 e = jj == i ? make_double2(1.0, 0.0) : cmul(cconj(xr[u]), scale);
+                                vi[jj] = e;
+                                vg[jj] = e;
+                                xr[u] = jj == i ? make_double2(beta, 0.0) : z;  // row i of L
+                            }
+                        }
+                        if (l32 == 0) {
+                            tauv[i] = tau;
+                            taus[r0 + i] = tau;
                         }
                     }
-                    if (l32 == 0) {
-                        tauv[i] = tau;
-                        taus[r0 + i] = tau;
-                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int s2 = sl; s2 < RPT; ++s2)
+                        if (team + kTeams * s2 > i) lq_reflect(x[s2], vl + i * S, i, n, l32, tauv[i]);
                 }
-                __syncthreads();
-                if (team > i) lq_reflect(x, vl + i * S, i, n, l32, tauv[i]);
-            }
-            lq_store_row(Wg, ld, row, r0, n, m, l32, x);
+#pragma unroll
+            for (int sl = 0; sl < RPT; ++sl) lq_store_row(Wg, ld, q * 16 + team + kTeams * sl, r0, n, m, l32, x[sl]);
         }
         sync();
         if (!live) continue;
@@ -984,10 +1020,13 @@ __device__ void lq_precondition(double2* __restrict__ Wg, int ld, int m, int c, 
                 __syncthreads();
                 staged = true;
             }
-            const int row = t * 16 + team;
-            lq_load_row<XL>(Wg, ld, row, r0, n, m, l32, x);
-            for (int i = 0; i < nb; ++i) lq_reflect(x, vl + i * S, i, n, l32, tauv[i]);
-            lq_store_row(Wg, ld, row, r0, n, m, l32, x);
+#pragma unroll
+            for (int sl = 0; sl < RPT; ++sl) lq_load_row<XL>(Wg, ld, t * 16 + team + kTeams * sl, r0, n, m, l32, x[sl]);
+            for (int i = 0; i < nb; ++i)
+#pragma unroll
+                for (int sl = 0; sl < RPT; ++sl) lq_reflect(x[sl], vl + i * S, i, n, l32, tauv[i]);
+#pragma unroll
+            for (int sl = 0; sl < RPT; ++sl) lq_store_row(Wg, ld, t * 16 + team + kTeams * sl, r0, n, m, l32, x[sl]);
         }
         __syncthreads();  // the next panel rewrites the LDS reflectors
     }
@@ -1013,7 +1052,8 @@ __device__ bool gram_converged(const double2* __restrict__ Wg, int ld, int m, in
                                double* __restrict__ nrm, int* vflag, int me, int nworkers, Sync& sync,
                                bool compute) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nw = nworkers * (kDistThreads / 64), gw = me * (kDistThreads / 64) + wave;
+    const int wpg = (int)(blockDim.x >> 6);
+    const int nw = nworkers * wpg, gw = me * wpg + wave;
     if (!compute) c = 0;  // only the barriers (lockstep: another group checks)
     for (int col = gw; col < c; col += nw) {  // exact norms^2
         const double2* x = Wg + (std::size_t)col * ld;
@@ -1175,12 +1215,12 @@ __device__ inline void wait_counter(int* ctr, int target, int* err, long long li
 // reading them. Accounting goes to this workgroup's slots: stats[4*b + 0..3] =
 // calls, sweeps, visited and rotated pair-rows, trunc[b] = max discarded weight
 // (b = blockIdx.x; the host reduces the slots, no cross-workgroup atomics).
-template <class Sync, bool XL = false>
+template <class Sync, bool XL = false, int RPT = 1>
 __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, int chi, double* __restrict__ sv_out,
                            int* __restrict__ perm_out, double* trunc, int* flags, unsigned long long* stats, int me,
                            int nworkers, Sync& sync, double2* lds, SvdShared& sh, const SvdMode& mode,
                            double2* aux, double2* taus) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = kDistThreads / RPT / 64;
     const int Dl = mode.active ? dims[p] : 1, Dr = mode.active ? dims[p + 2] : 1;
     const int R = 2 * Dl, C = 2 * Dr;
     const bool tall = C <= R;
@@ -1203,7 +1243,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     // summed in column order by every wave, so every worker and every layout gets
     // the same bits
     if (mode.active)
-        for (int col = me * (kDistThreads / 64) + wave; col < c; col += nworkers * (kDistThreads / 64)) {
+        for (int col = me * nwaves + wave; col < c; col += nworkers * nwaves) {
             const double2* x = Wg + (std::size_t)col * ld;
             double a = 0.0;
             for (int row = lane; row < m; row += 64) a += cnorm(gld<XL>(x + row));
@@ -1222,7 +1262,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     unsigned long long tp = prof ? wall_clock64() : 0;
     if (aux) {  // W <- L of W = L Q^H (same left singular vectors and values)
         const int panels = mode.lq_panels >= 0 ? mode.lq_panels : (c + 15) / 16;
-        lq_precondition<Sync, XL>(Wg, ld, m, c, aux, taus, me, nworkers, sync, lds, panels, mode.active);
+        lq_precondition<Sync, XL, RPT>(Wg, ld, m, c, aux, taus, me, nworkers, sync, lds, panels, mode.active);
     }
     if (prof) {
         const unsigned long long t = wall_clock64();
@@ -1240,12 +1280,14 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     // row layout, at most one pair per worker and P a power of two: the
     // resident-block tournament (tour_pair), else rr_pair. The ordering depends
     // only on the shape, so every synchronisation mode applies the same rotations.
-    const bool tour = mode.rows && nworkers >= P && (P & (P - 1)) == 0;
-    double2 rv[16];  // the row layout's registers, kept across rounds for a resident block
+    // (RPT > 1: always the row layout; the LDS-column rounds need 8 waves)
+    const bool rows = RPT > 1 || mode.rows;
+    const bool tour = rows && nworkers >= P && (P & (P - 1)) == 0;
+    double2 rv[RPT][16];  // the row layout's registers, kept across rounds for a resident block
     int ep = p2p ? __hip_atomic_load(&mode.epochs[(std::size_t)me * 64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     int sweeps_done = 0, ph = 0;
     unsigned long long nvis = 0, nrot = 0;
-    const bool gram = mode.gram && mode.rows && mode.stop_ratio == 0.0;
+    const bool gram = mode.gram && rows && mode.stop_ratio == 0.0;
     // a check pays off once a sweep rotates at most about 1/256 of the pairs
     const int gram_rot = max(16, c * (c - 1) / 512);
     for (int sweep = 0; sweep < 40; ++sweep) {
@@ -1313,9 +1355,9 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                 };
                 int rflags = 0, wrot = 0;
                 tpost = 0;
-                if (mode.rows) {
+                if (rows) {
                     int nv = 0, nr = 0;
-                    rflags = round_rows<XL>(Wg, ld, m, c, bi, bj, r == 0, floor2, mode.stop_ratio,
+                    rflags = round_rows<XL, RPT>(Wg, ld, m, c, bi, bj, r == 0, floor2, mode.stop_ratio,
                                             *reinterpret_cast<RowLds*>(lds), ph, nv, nr,
                                             (prof && mode.step_prof) ? sh.tm + kProfStep : nullptr, rv, load_a,
                                             store_a, lap);
@@ -1326,7 +1368,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
                         wrot = nr;
                         srot += nr;
                     }
-                } else {
+                } else if constexpr (RPT == 1) {
                 double2 xa[kRowsPerLane], ya[kRowsPerLane];
                 // LDS column q <-> block (q < 8 ? bi : bj), member q & 7; a column's
                 // rows go through registers with all of a lane's loads in flight at
@@ -1505,7 +1547,7 @@ __device__ void svd_blocks(double2* __restrict__ Wg, int ld, int* dims, int p, i
     }
     // column norms -> sv_out (scratch), then the leader sorts and truncates
     if (mode.active)
-        for (int col = me * (kDistThreads / 64) + wave; col < c; col += nworkers * (kDistThreads / 64)) {
+        for (int col = me * nwaves + wave; col < c; col += nworkers * nwaves) {
             const double2* x = Wg + (std::size_t)col * ld;
             double s = 0.0;
             for (int row = lane; row < m; row += 64) s += cnorm(gld<XL>(x + row));
@@ -2282,10 +2324,19 @@ struct BlockSync {
 // barrier), kExecGroupSync (grouped, each group's update on the group's own
 // barrier, GroupSync), kExecSolo (one workgroup per update).
 constexpr int kExecLockstep = 0, kExecGroupSync = 1, kExecSolo = 2;
+// Rows of a Jacobi round per thread in the grouped layouts (workgroups of 512 /
+// kExecRows threads; the solo layout always runs 512 threads with one row each).
+// The row layout, LQ and convergence check are written for 1 or 2 rows per
+// thread, but 2 is switched off: its first run (n = 12, chi = 64, eight workers
+// per group) ended in an illegal memory access whose cause has not been found.
+constexpr int kExecRows = 1;
+__host__ __device__ constexpr int exec_threads(int layout) {
+    return layout == kExecSolo ? kDistThreads : kDistThreads / kExecRows;
+}
 // XL (grouped layout only): the groups sit one per XCD and synchronise
 // XCD-locally (GroupSync<true>, L1-bypassing loads of the group's data).
 template <int LAYOUT, bool XL = false>
-__global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
+__global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs a) {
     extern __shared__ double2 lds[];
     __shared__ SvdShared sh;
     const int tid = threadIdx.x;
@@ -2413,7 +2464,7 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
                 mode.err = a.err;
                 mode.spin_limit = a.spin_limit;
                 mode.naps = a.naps;
-                svd_blocks<GroupSync<XL>, XL>(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, rank,
+                svd_blocks<GroupSync<XL>, XL, kExecRows>(W, a.ld, I.dims, u.site, I.chi, sv, perm, I.trunc, sflags, I.stats, rank,
                                               nwg, gs, lds, sh, mode, (a.aux ? a.aux + grp * mat_sz : nullptr),
                                               a.taus + (std::size_t)grp * a.ld);
                 mark(3);
@@ -2450,7 +2501,8 @@ __global__ void __launch_bounds__(kDistThreads) k_tebd_lockstep(ExecArgs a) {
             mode.active = active;
             mode.lq_panels = (a.ld + 15) / 16;
             mode.prof = a.prof_on && blockIdx.x == 0;
-            svd_blocks(W, a.ld, I.dims, active ? u.site : 0, I.chi, sv, perm, I.trunc, sflags, I.stats, rank, nwg, sync,
+            svd_blocks<GridSync, false, kExecRows>(W, a.ld, I.dims, active ? u.site : 0, I.chi, sv, perm, I.trunc, sflags,
+                                                   I.stats, rank, nwg, sync,
                        lds, sh, mode, (a.aux ? a.aux + grp * mat_sz : nullptr), a.taus + (std::size_t)grp * a.ld);
             mark(3);
             sync();
@@ -3006,8 +3058,8 @@ static void exec_shape(int maxcap, int req, bool solo, bool group_sync, int work
     int per_cu = 0, cus = 0, dev = 0;
     TQ_HIP(hipGetDevice(&dev));
     TQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, exec_kernel(solo, group_sync, xl), kDistThreads,
-                                                        shm));
+    TQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, exec_kernel(solo, group_sync, xl),
+                                                        exec_threads(solo ? kExecSolo : kExecLockstep), shm));
     const long long resident = std::min<long long>((long long)per_cu * cus, kStatSlots);
     long long g = resident / per_group;
     const long long cap = req > 0 ? req : (solo ? kStatSlots : 64);
@@ -3357,7 +3409,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     }
     if (total_steps > 0)
         TQ_HIP(hipLaunchCooperativeKernel(exec_kernel(solo, group_sync, xl),
-                                          dim3((unsigned)(G * W)), dim3(kDistThreads),
+                                          dim3((unsigned)(G * W)), dim3(exec_threads(solo ? kExecSolo : kExecLockstep)),
                                           args, shm, st));
     if (t0.timing_) TQ_HIP(hipEventRecord(t0.events_[t0.ev_next_++].second, st));
     t0.exec_supersteps_ += total_steps;

@@ -768,7 +768,7 @@ __device__ __forceinline__ void row_step(double2 (&v)[RPT][16], RowLds& rl, int&
     const unsigned long long needm = __ballot(need) & even, vism = __ballot(vis) & even;
     wave_lds_fence();
     mark(3);
-    if (rotm) {
+    {  // unconditional: a branch around it costs register copies at the join
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const double c0 = rl.par[wave][q][0], s0 = rl.par[wave][q][1];

@@ -840,7 +840,13 @@ __device__ __forceinline__ int round_rows(double2* __restrict__ Wg, int ld, int 
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const int row = tid + r * kThreads;
-            if ((q >= kBlk || store_a) && ((cvalid >> q) & 1) && row < m) Wg[(std::size_t)col * ld + row] = v[r][q];
+            if ((q >= kBlk || store_a) && ((cvalid >> q) & 1) && row < m) {
+                // device-scope stores write through the L2, so the round's release
+                // finds no dirty lines of the block to write back
+                double* dst = reinterpret_cast<double*>(Wg + (std::size_t)col * ld + row);
+                __hip_atomic_store(dst, v[r][q].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(dst + 1, v[r][q].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
     lap(15);

@@ -640,6 +640,13 @@ __host__ __device__ constexpr int row_pair_b(int kind, int s, int k) {
     return kind ? 8 + ((k + s) & 7) : (k & 3) == 0 ? 7 + (k & 4) * 2 : (s - (k & 3) + 7) % 7 + (k & 4) * 2;
 }
 
+// Raw buffer stores (__builtin_amdgcn_raw_buffer_store_b128): the fourth word of
+// the buffer resource (32-bit data format, no swizzle) and the cache-policy bit
+// that writes a store through the L2 (sc1).
+typedef int v4i32 __attribute__((ext_vector_type(4)));
+constexpr int kBufferRsrcWord3 = 0x00020000;
+constexpr int kCpolSc1 = 16;
+
 // Orders this wave's LDS writes before its later LDS reads by other lanes.
 __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -813,6 +820,14 @@ __device__ __forceinline__ int round_rows(double2* __restrict__ Wg, int ld, int 
                                  Lap&& lap) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int kThreads = kDistThreads / RPT;  // row tid + r * kThreads in v[r]
+    // the write-back's buffer resource: Wg's ld x ld matrix (Wg and ld are uniform;
+    // read from lane 0 so that the resource sits in scalar registers)
+    const unsigned long long wa = reinterpret_cast<unsigned long long>(Wg);
+    const unsigned long long wu = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(wa >> 32)) << 32) |
+                                  (unsigned)__builtin_amdgcn_readfirstlane((int)wa);
+    const int wbytes = __builtin_amdgcn_readfirstlane((int)((std::size_t)ld * ld * sizeof(double2)));
+    const __amdgpu_buffer_rsrc_t wrsrc =
+        __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(wu), 0, wbytes, kBufferRsrcWord3);
     unsigned cvalid = 0;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -841,11 +856,16 @@ __device__ __forceinline__ int round_rows(double2* __restrict__ Wg, int ld, int 
         for (int r = 0; r < RPT; ++r) {
             const int row = tid + r * kThreads;
             if ((q >= kBlk || store_a) && ((cvalid >> q) & 1) && row < m) {
-                // device-scope stores write through the L2, so the round's release
-                // finds no dirty lines of the block to write back
-                double* dst = reinterpret_cast<double*>(Wg + (std::size_t)col * ld + row);
-                __hip_atomic_store(dst, v[r][q].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(dst + 1, v[r][q].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if constexpr (XL) {
+                    Wg[(std::size_t)col * ld + row] = v[r][q];
+                } else {
+                    // 16-byte stores with sc1 write through the L2, so the round's
+                    // release finds no dirty lines of the block to write back
+                    v4i32 d;
+                    __builtin_memcpy(&d, &v[r][q], sizeof(d));
+                    __builtin_amdgcn_raw_buffer_store_b128(d, wrsrc, (int)(((std::size_t)col * ld + row) * sizeof(double2)),
+                                                           0, kCpolSc1);
+                }
             }
         }
     }

@@ -2320,14 +2320,21 @@ struct GroupSync {
                     __hip_atomic_store(bar, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     __hip_atomic_fetch_add(bar + 1, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 } else {
+                    // relaxed polls (an acquire poll would invalidate the XCD's L2 each
+                    // time), then one acquire fence that synchronises with the last
+                    // arrival's release; the error word is read every 32nd poll
+                    bool gave_up = false;
                     for (long long spins = 0;
-                         __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen; ++spins) {
-                        if (spins >= limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                         __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen; ++spins) {
+                        if (spins >= limit ||
+                            ((spins & 31) == 31 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                             __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            gave_up = true;
                             break;
                         }
                         for (int q = 0; q < 2 * naps; ++q) __builtin_amdgcn_s_sleep(1);
                     }
+                    if (!gave_up) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 }
             }
         }

@@ -571,7 +571,9 @@ def sweep_summary(h0, h1, rounds):
     out = {"svds": [int(x) for x in svds[:last]],
            "rotations_per_svd": [round(float(r / v), 1) if v else 0.0 for r, v in zip(rot[:last], svds[:last])],
            "rounds_ms": round(h1["rounds_ms"] - h0["rounds_ms"], 3),
-           "sweep_end_ms": round(h1["sweep_end_ms"] - h0["sweep_end_ms"], 3)}
+           "sweep_end_ms": round(h1["sweep_end_ms"] - h0["sweep_end_ms"], 3),
+           "round_pre_ms": round(h1["round_pre_ms"] - h0["round_pre_ms"], 3),
+           "round_post_ms": round(h1["round_post_ms"] - h0["round_post_ms"], 3)}
     if any(cyc.values()):
         out["step_cycles_per_round"] = cyc
     return out

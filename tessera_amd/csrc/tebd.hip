@@ -468,7 +468,10 @@ constexpr int kStatSlots = 1024;   // per-workgroup accounting slots (largest gr
 constexpr int kHistSweeps = 24;  // sweep histogram: sweeps past the last fold into it
 constexpr int kProfHist = 17;    // first histogram slot
 constexpr int kProfStep = kProfHist + 3 * kHistSweeps;  // row-layout step parts (cycles)
-constexpr int kProfRound = kProfStep + 5;  // whole rounds, sweep ends (wall ticks)
+// wall ticks: whole rounds, sweep ends, the part of a round before its pair work
+// (the producer wait included), the part between the pair work and the publication
+// (the write-back's stores completing)
+constexpr int kProfRound = kProfStep + 5;
 constexpr int kProfSlots = kProfRound + 4;
 
 struct SvdShared {

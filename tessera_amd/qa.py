@@ -481,15 +481,19 @@ class TebdSession:
         stop ratio (every rotating round when the ratio is 0), `svds` its SVDs
         that ran the sweep; `step_cycles`: its thread 0's shader cycles in the
         parts of the row-layout steps; `rounds_ms`: its time in whole tournament
-        rounds, `sweep_end_ms` in the flag checks between sweeps."""
+        rounds, `sweep_end_ms` in the flag checks between sweeps; `round_pre_ms`:
+        the part of its rounds before the pair work (the producer waits included),
+        `round_post_ms` the part between the pair work and the publication (the
+        write-back's stores completing)."""
         n = len(self.PROFILE_SLOTS)
         k = n + 3 * self.HIST_SWEEPS
-        out = np.zeros(k + 5 + 2)
+        out = np.zeros(k + 5 + 4)
         _check(lib().tessera_tebd_exec_profile(self._h, _p(out), out.size))
         h = out[n:k].reshape(3, self.HIST_SWEEPS)
         steps = dict(zip(("dots_reduce", "barrier", "rotation", "broadcast", "apply"), out[k:k + 5].tolist()))
         return {"rotations": h[0], "big_rounds": h[1], "svds": h[2], "step_cycles": steps,
-                "rounds_ms": float(out[k + 5]), "sweep_end_ms": float(out[k + 6])}
+                "rounds_ms": float(out[k + 5]), "sweep_end_ms": float(out[k + 6]),
+                "round_pre_ms": float(out[k + 7]), "round_post_ms": float(out[k + 8])}
 
     @property
     def updates_per_step(self) -> int:

@@ -1001,7 +1001,8 @@ __device__ void lq_precondition(double2* __restrict__ Wg, int ld, int m, int c, 
                             const double rdd = 1.0 / (d.x * d.x + d.y * d.y);
                             scale = make_double2(d.x * rdd, -d.y * rdd);
                         }
-                        double2* vg = aux + (std::size_t)(r0 + i) * ld + r0;
+                        // into LDS only: a global store here would hold up the barrier
+                        // below (it waits for the workgroup's outstanding stores)
                         double2* vi = vl + i * S;
 #pragma unroll
                         for (int u = 0; u < kLqCols; ++u) {
@@ -1009,14 +1010,10 @@ __device__ void lq_precondition(double2* __restrict__ Wg, int ld, int m, int c, 
                             if (jj >= i && jj < n) {
                                 const double2 e = jj == i ? make_double2(1.0, 0.0) : cmul(cconj(xr[u]), scale);
                                 vi[jj] = e;
-                                vg[jj] = e;
                                 xr[u] = jj == i ? make_double2(beta, 0.0) : z;  // row i of L
                             }
                         }
-                        if (l32 == 0) {
-                            tauv[i] = tau;
-                            taus[r0 + i] = tau;
-                        }
+                        if (l32 == 0) tauv[i] = tau;
                     }
                     __syncthreads();
 #pragma unroll
@@ -1025,6 +1022,13 @@ __device__ void lq_precondition(double2* __restrict__ Wg, int ld, int m, int c, 
                 }
 #pragma unroll
             for (int sl = 0; sl < RPT; ++sl) lq_store_row(Wg, ld, q * 16 + team + kTeams * sl, r0, n, m, l32, x[sl]);
+            // the panel's reflectors (LDS, complete after the last barrier above) and
+            // taus to aux and taus for the other workers
+            for (int idx = tid; idx < nb * n; idx += (int)blockDim.x) {
+                const int k = idx / n, jj = idx - k * n;
+                if (jj >= k) aux[(std::size_t)(r0 + k) * ld + r0 + jj] = vl[k * S + jj];
+            }
+            if (tid < nb) taus[r0 + tid] = tauv[tid];
         }
         sync();
         if (!live) continue;

@@ -2667,7 +2667,7 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     ex_dataflow_ = df && df[0] == '1';
     // knob: 64-cycle sleeps between the polls of a group barrier or round wait
     const char* nap = std::getenv("TESSERA_EXEC_POLL_NAPS");
-    ex_naps_ = nap ? std::max(1, std::atoi(nap)) : 1;
+    ex_naps_ = nap ? std::max(0, std::atoi(nap)) : 1;
     // knob: the Jacobi sweeps of the executor's SVDs end after a sweep whose
     // rotations all had a relative coupling of at most this value (0: after a
     // sweep that rotates nothing)

@@ -14,8 +14,11 @@ Workload (one "step" = one symmetric Trotter step of the MPS-TEBD anneal):
 By default the timed steps (6) run in the lockstep executor: one cooperative
 launch in which consecutive Trotter steps overlap (1874 of a step's 1998 updates
 form one dependency chain, so a launch of K steps has ~K updates per super-step
-while the pipeline fills; 6 steps give 5.46 on average, a 1000-step anneal in
-launches of 64 steps 7.5). --executor 0 runs one launch sequence per update.
+of the host's schedule while the pipeline fills; 6 steps give 5.46 on average, a
+1000-step anneal in launches of 64 steps 7.5). Each group runs its updates in that
+order and waits only for the done flags of each update's predecessors on its two
+sites (the dataflow order; TESSERA_EXEC_DATAFLOW=0 adds a grid barrier after every
+super-step). --executor 0 runs one launch sequence per update.
 
 Output: one JSON line. `value` is two-site updates per second over the timed
 steps (inputs resident on the device). `roofline` covers the dominant kernel
@@ -629,8 +632,8 @@ def tebd_timed(args, model, sched, k0, sess):
         ex1 = sess.exec_stats()
         ss = ex1["supersteps"] - ex0["supersteps"]
         kname = "k_tebd_lockstep"
-        kernel = ("k_tebd_lockstep (super-steps of theta and reconstruction on FP64 MFMA, block-Jacobi SVDs "
-                  "(FP64 VALU) on per-group barriers with point-to-point tournament rounds)")
+        kernel = ("k_tebd_lockstep (updates in dataflow order: theta and reconstruction on FP64 MFMA, block-Jacobi "
+                  "SVDs (FP64 VALU) on per-group barriers with point-to-point tournament rounds)")
         pr1 = sess.exec_profile()
         # the theta and reconstruction phases alone (workgroup 0's timers cover its
         # group's updates): F of those updates / phase time, against the FP64 MFMA

@@ -2276,12 +2276,39 @@ struct ExecArgs {
     int step_prof;         // SvdMode::step_prof of the profiled workgroup
     int prof_on;           // workgroup 0 records the phase profile
     int* epochs;        // [G][W][64] round counters of the point-to-point rounds (per-group layout)
-    int dataflow;       // per-group layout without snapshots: no super-step barriers; an update
-                        // waits for the done flags of its predecessors instead
+    int dataflow;       // per-group layout: no super-step barriers; an update waits for the
+                        // done flags of its predecessors instead
     int* done;          // [updates] done flags of the dataflow order
+    // dataflow order: snaps[0, npro) are taken before any update (their sites have
+    // none earlier in the launch); snaps[usnap_begin[f], usnap_begin[f + 1]) by
+    // update f after its reconstruction (it is the latest update on their site)
+    const int* usnap_begin;  // [updates + 1]
+    int npro;
     unsigned long long* prof;  // [kProfSlots] phase profile of workgroup 0 (accumulated)
     unsigned long long* ss_clock;  // [nsteps + 1] or null: wall clock at the end of each super-step, then at the start
 };
+
+// Site snapshot of instance 0 for the entropy trace: site i with the pending
+// single-site gate applied, into its slot (threads idx0, idx0 + stride, ...).
+__device__ void take_snapshot(const ExecSnap& sn, const ExecInst& I0, double2* snap, std::size_t snap_stride,
+                              int* snap_dims, int n, long long idx0, long long stride) {
+    const int i = sn.site;
+    const int Dl = I0.dims[i], Dr = I0.dims[i + 1], stc = I0.cap[i + 1];
+    const double2* src = I0.sites + I0.off[i];
+    double2* dst = snap + (std::size_t)sn.slot * snap_stride + I0.off[i];
+    for (long long idx = idx0; idx < (long long)Dl * Dr; idx += stride) {
+        const int x = (int)(idx / Dr), b = (int)(idx % Dr);
+        const double2 v0 = src[((std::size_t)x * 2 + 0) * stc + b];
+        const double2 v1 = src[((std::size_t)x * 2 + 1) * stc + b];
+        dst[((std::size_t)x * 2 + 0) * stc + b] = cadd(cmul(sn.g[0], v0), cmul(sn.g[1], v1));
+        dst[((std::size_t)x * 2 + 1) * stc + b] = cadd(cmul(sn.g[2], v0), cmul(sn.g[3], v1));
+    }
+    if (idx0 == 0) {
+        int* sd = snap_dims + (std::size_t)sn.slot * (n + 1);
+        sd[i] = Dl;
+        sd[i + 1] = Dr;
+    }
+}
 
 // Barrier of the nwg co-resident workgroups of one executor group (the
 // cooperative launch keeps every workgroup resident). bar[0] counts arrivals,
@@ -2290,6 +2317,7 @@ struct ExecArgs {
 // when another workgroup has raised *err, or after kSpinLimit polls, which
 // raise *err (the host reports it) instead of hanging the device.
 constexpr long long kSpinLimit = 1ll << 24;
+constexpr long long kDoneWaitFactor = 64;
 template <bool XL = false>
 struct GroupSync {
     int* bar;
@@ -2423,30 +2451,19 @@ __global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs
         }
     };
     const ExecInst& I0 = a.insts[0];
+    if (a.dataflow && a.npro > 0) {  // snapshots of sites no update of the launch touches first
+        for (int q = 0; q < a.npro; ++q) take_snapshot(a.snaps[q], I0, a.snap, a.snap_stride, a.snap_dims, a.n, tid_all, all);
+        sync();
+    }
     for (int st = 0; st < a.nsteps; ++st) {
         const bool active = grp < a.up_begin[st + 1] - a.up_begin[st];
         const ExecUpdate& u = a.ups[active ? a.up_begin[st] + grp : 0];
         const ExecInst& I = a.insts[active ? u.inst : 0];
-        // 1. snapshots (whole grid, instance 0), theta (group)
-        for (int q = a.snap_begin[st]; q < a.snap_begin[st + 1]; ++q) {
-            const ExecSnap& sn = a.snaps[q];
-            const int i = sn.site;
-            const int Dl = I0.dims[i], Dr = I0.dims[i + 1], stc = I0.cap[i + 1];
-            const double2* src = I0.sites + I0.off[i];
-            double2* dst = a.snap + (std::size_t)sn.slot * a.snap_stride + I0.off[i];
-            for (long long idx = tid_all; idx < (long long)Dl * Dr; idx += all) {
-                const int x = (int)(idx / Dr), b = (int)(idx % Dr);
-                const double2 v0 = src[((std::size_t)x * 2 + 0) * stc + b];
-                const double2 v1 = src[((std::size_t)x * 2 + 1) * stc + b];
-                dst[((std::size_t)x * 2 + 0) * stc + b] = cadd(cmul(sn.g[0], v0), cmul(sn.g[1], v1));
-                dst[((std::size_t)x * 2 + 1) * stc + b] = cadd(cmul(sn.g[2], v0), cmul(sn.g[3], v1));
-            }
-            if (tid_all == 0) {
-                int* sd = a.snap_dims + (std::size_t)sn.slot * (a.n + 1);
-                sd[i] = Dl;
-                sd[i + 1] = Dr;
-            }
-        }
+        // 1. snapshots (whole grid, instance 0; the dataflow order takes them with
+        // the updates), theta (group)
+        if (!a.dataflow)
+            for (int q = a.snap_begin[st]; q < a.snap_begin[st + 1]; ++q)
+                take_snapshot(a.snaps[q], I0, a.snap, a.snap_stride, a.snap_dims, a.n, tid_all, all);
         if constexpr (LAYOUT == kExecSolo) {
             // the snapshots read sites that this super-step's reconstructions may rewrite
             if (a.snap_begin[st + 1] > a.snap_begin[st]) sync();
@@ -2483,7 +2500,12 @@ __global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs
                     // updates on them are done (each of those waited for its own)
                     if (tid == 0) {
                         for (int q = 0; q < 2; ++q)
-                            if (u.pred[q] >= 0) wait_counter(&a.done[u.pred[q]], 1, a.err, a.spin_limit);
+                            // a group with no update in the super-steps before this one
+                            // arrives here early and may wait as long as the critical path
+                            // ahead of it (seconds at n = 128, chi = 256), so this wait
+                            // allows kDoneWaitFactor times the polls of a barrier wait
+                            if (u.pred[q] >= 0)
+                                wait_counter(&a.done[u.pred[q]], 1, a.err, a.spin_limit * kDoneWaitFactor);
                         // each wait ends with an acquire fence: the predecessors' sites are visible
                     }
                     __syncthreads();
@@ -2515,6 +2537,13 @@ __global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs
                 if (prof) sh.tm[7] += 1;
                 if (a.dataflow) {
                     gs();  // every workgroup's part of the reconstruction is released
+                    const int f = a.up_begin[st] + grp;
+                    if (a.usnap_begin[f + 1] > a.usnap_begin[f]) {
+                        // the snapshots of the sites this update leaves final for a step
+                        for (int q = a.usnap_begin[f]; q < a.usnap_begin[f + 1]; ++q)
+                            take_snapshot(a.snaps[q], I0, a.snap, a.snap_stride, a.snap_dims, a.n, gid, gstride);
+                        gs();  // read before a later update on the sites may rewrite them
+                    }
                     if (rank == 0 && tid == 0) {
                         __hip_atomic_store(&a.done[a.up_begin[st] + grp], 1, __ATOMIC_RELEASE,
                                            __HIP_MEMORY_SCOPE_AGENT);
@@ -2661,10 +2690,12 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     // knob: 0 keeps the per-group layout on a full barrier after every round
     const char* pp = std::getenv("TESSERA_EXEC_P2P");
     ex_p2p_ = !(pp && pp[0] == '0');
-    // knob: 1 drops the super-step barriers of the per-group layout (dataflow order:
-    // same results; no faster on the n = 128 chain, slower on mixed-chi batches)
+    // knob: the dataflow order of the per-group layout (no super-step barriers, same
+    // results): 1 always, 0 never; unset: for single-instance runs (9% faster on
+    // the n = 128 chain), while batched runs keep the super-steps (their per-instance
+    // completion times come from the super-step clocks)
     const char* df = std::getenv("TESSERA_EXEC_DATAFLOW");
-    ex_dataflow_ = df && df[0] == '1';
+    ex_dataflow_ = df ? (df[0] == '1' ? 1 : 0) : -1;
     // knob: 64-cycle sleeps between the polls of a group barrier or round wait
     const char* nap = std::getenv("TESSERA_EXEC_POLL_NAPS");
     ex_naps_ = nap ? std::max(0, std::atoi(nap)) : 1;
@@ -3137,9 +3168,12 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     int G = 0, W = 0;
     std::size_t shm = 0;
     const bool group_sync = !solo && t0.ex_group_sync_;
+    // dataflow order (per-group layout, no super-step barriers): every snapshot is
+    // taken by the latest update on its site, or before any update when there is none
+    const bool dataflow = group_sync && (t0.ex_dataflow_ == 1 || (t0.ex_dataflow_ < 0 && ts.size() == 1));
     // XCD-local groups: grouped layout, one group per XCD (G a multiple of 8),
     // not in the dataflow order (whose done flags cross XCDs)
-    bool xl = group_sync && t0.ex_xcd_local_ && !t0.ex_dataflow_ && xcd_round_robin();
+    bool xl = group_sync && t0.ex_xcd_local_ && !dataflow && xcd_round_robin();
     int Gxl = 0, Wxl = 0;
     std::size_t shm_xl = 0;
     if (xl) {
@@ -3168,6 +3202,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     std::vector<int> rounds;
     struct SnapAt {
         int at;
+        long long pos;  // (super-step << 16 | slot) of the site's latest update, -1: none
         ExecSnap s;
     };
     std::vector<SnapAt> snaps;
@@ -3283,6 +3318,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
                     // the reconstruction phase), before any later update changes it
                     SnapAt x;
                     x.at = last_at[i] + 1;
+                    x.pos = last_pos[i];
                     x.s.site = i;
                     x.s.slot = k;
                     x.s.g[0] = to_d2(pending[i].u[0][0]);
@@ -3312,14 +3348,36 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
         W = Wxl;
         shm = shm_xl;
     }
-    std::stable_sort(snaps.begin(), snaps.end(), [](const SnapAt& x, const SnapAt& y) { return x.at < y.at; });
+    const int nup = up_begin[total_steps];
     std::vector<int> snap_begin(total_steps + 1, 0);
+    std::vector<int> usnap_begin;  // dataflow: [nup + 1]
+    int npro = 0;
     std::vector<ExecSnap> snap_list(snaps.size());
-    for (std::size_t q = 0; q < snaps.size(); ++q) {
-        snap_list[q] = snaps[q].s;
-        ++snap_begin[snaps[q].at + 1];
+    if (dataflow) {
+        auto owner = [&](const SnapAt& x) {  // flat index of the update taking it, -1: before any
+            return x.pos < 0 ? -1 : up_begin[(int)(x.pos >> 16)] + (int)(x.pos & 0xffff);
+        };
+        std::stable_sort(snaps.begin(), snaps.end(),
+                         [&](const SnapAt& x, const SnapAt& y) { return owner(x) < owner(y); });
+        usnap_begin.assign((std::size_t)nup + 1, 0);
+        for (std::size_t q = 0; q < snaps.size(); ++q) {
+            snap_list[q] = snaps[q].s;
+            const int f = owner(snaps[q]);
+            if (f < 0)
+                ++npro;
+            else
+                ++usnap_begin[f + 1];
+        }
+        usnap_begin[0] = npro;
+        for (int f = 0; f < nup; ++f) usnap_begin[f + 1] += usnap_begin[f];
+    } else {
+        std::stable_sort(snaps.begin(), snaps.end(), [](const SnapAt& x, const SnapAt& y) { return x.at < y.at; });
+        for (std::size_t q = 0; q < snaps.size(); ++q) {
+            snap_list[q] = snaps[q].s;
+            ++snap_begin[snaps[q].at + 1];
+        }
+        for (int t = 0; t < total_steps; ++t) snap_begin[t + 1] += snap_begin[t];
     }
-    for (int t = 0; t < total_steps; ++t) snap_begin[t + 1] += snap_begin[t];
     // a super-step whose slots are all idle still runs one (empty) round
     for (int t = 0; t < total_steps; ++t) rounds[t] = std::max(rounds[t], 1);
 
@@ -3355,7 +3413,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     const std::size_t in_bytes = align(insts.size() * sizeof(ExecInst));
     const std::size_t up_bytes = align(std::max<std::size_t>(1, up_begin[total_steps]) * sizeof(ExecUpdate));
     const std::size_t sn_bytes = align(std::max<std::size_t>(1, snap_list.size()) * sizeof(ExecSnap));
-    const std::size_t int_count = (std::size_t)total_steps + 2 * (total_steps + 1);
+    const std::size_t int_count = (std::size_t)total_steps + 2 * (total_steps + 1) + usnap_begin.size();
     t0.ex_host_.assign(in_bytes + up_bytes + sn_bytes + int_count * sizeof(int), 0);
     unsigned char* hb = t0.ex_host_.data();
     std::memcpy(hb, insts.data(), insts.size() * sizeof(ExecInst));
@@ -3375,6 +3433,8 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     std::memcpy(hi, rounds.data(), total_steps * sizeof(int));
     std::memcpy(hi + total_steps, snap_begin.data(), (total_steps + 1) * sizeof(int));
     std::memcpy(hi + 2 * total_steps + 1, up_begin.data(), (total_steps + 1) * sizeof(int));
+    if (!usnap_begin.empty())
+        std::memcpy(hi + 3 * total_steps + 2, usnap_begin.data(), usnap_begin.size() * sizeof(int));
     if (t0.ex_tasks_.n < t0.ex_host_.size()) t0.ex_tasks_.alloc(t0.ex_host_.size());
     TQ_HIP(hipMemcpy(t0.ex_tasks_.p, hb, t0.ex_host_.size(), hipMemcpyHostToDevice));
 
@@ -3385,6 +3445,8 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.rounds = reinterpret_cast<const int*>(t0.ex_tasks_.p + in_bytes + up_bytes + sn_bytes);
     x.snap_begin = x.rounds + total_steps;
     x.up_begin = x.rounds + 2 * total_steps + 1;
+    x.usnap_begin = usnap_begin.empty() ? nullptr : x.rounds + 3 * total_steps + 2;
+    x.npro = npro;
     x.small = solo ? 1 : 0;
     x.nsteps = total_steps;
     x.G = G;
@@ -3409,7 +3471,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.gram = t0.svd_gram_ ? 1 : 0;
     x.step_prof = t0.ex_step_prof_ ? 1 : 0;
     x.prof_on = t0.ex_prof_on_ ? 1 : 0;
-    x.dataflow = (group_sync && t0.ex_dataflow_ && snap_list.empty()) ? 1 : 0;
+    x.dataflow = dataflow ? 1 : 0;
     x.done = nullptr;
     if (x.dataflow) {
         const std::size_t nu = std::max<std::size_t>(1, up_begin[total_steps]);

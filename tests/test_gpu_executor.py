@@ -279,23 +279,25 @@ def test_group_barrier_wait_that_gives_up_is_reported(monkeypatch):
     assert np.isfinite(g["max_trunc"]) and g["bonds"].max() <= 32
 
 
-@pytest.mark.parametrize("chi", [32, 8])
-def test_dataflow_order_matches_superstep_barriers(monkeypatch, chi):
-    """With TESSERA_EXEC_DATAFLOW=1 and no entropy recording the per-group
-    layout drops the super-step barriers: each update waits only for the done
-    flags of the latest earlier updates on its two sites (every dependency, bond
-    dimensions included, goes through a shared site). The same rotations run on
-    the same data, so the run equals the default one with super-step barriers
-    and the lockstep one: a range-3 chain of 14 sites, 5 steps in one launch,
-    exact at chi = 32 and truncated at chi = 8."""
+@pytest.mark.parametrize("chi,record", [(32, False), (8, False), (32, True), (8, True)])
+def test_dataflow_order_matches_superstep_barriers(monkeypatch, chi, record):
+    """In the dataflow order (the default for one instance; TESSERA_EXEC_DATAFLOW=0
+    keeps the super-step barriers) the per-group layout has no super-step
+    barriers: each update waits only for the done flags of the latest earlier
+    updates on its two sites (every dependency, bond dimensions included, goes
+    through a shared site), and each entropy snapshot is taken by the latest
+    update on its site before it releases its done flag. The same rotations run
+    on the same data, so the run equals the one with super-step barriers and the
+    lockstep one, entropy trace included: a range-3 chain of 14 sites, 5 steps in
+    one launch, exact at chi = 32 and truncated at chi = 8."""
     model = I.chain_range(14, 3, 31)
     sched = I.linear(5)
-    steps = _run(model, sched, 0.1, chi, 8, record=False)
-    monkeypatch.setenv("TESSERA_EXEC_DATAFLOW", "1")
-    flow = _run(model, sched, 0.1, chi, 8, record=False)
+    monkeypatch.setenv("TESSERA_EXEC_DATAFLOW", "0")
+    steps = _run(model, sched, 0.1, chi, 8, record=record)
     monkeypatch.delenv("TESSERA_EXEC_DATAFLOW")
+    flow = _run(model, sched, 0.1, chi, 8, record=record)
     monkeypatch.setenv("TESSERA_EXEC_LOCKSTEP", "1")
-    lock = _run(model, sched, 0.1, chi, 8, record=False)
+    lock = _run(model, sched, 0.1, chi, 8, record=record)
     for name, other in (("super-step barriers", steps), ("lockstep", lock)):
         inf = 1 - _fid(flow["state"], other["state"])
         same = np.array_equal(flow["state"], other["state"])
@@ -305,3 +307,6 @@ def test_dataflow_order_matches_superstep_barriers(monkeypatch, chi):
         assert inf <= 1e-12
         assert abs(flow["max_trunc"] - other["max_trunc"]) <= 1e-12 + 1e-9 * other["max_trunc"]
         assert np.array_equal(flow["spins"], other["spins"])
+        assert flow["entropy"].shape == other["entropy"].shape == ((5,) if record else (0,))
+        if record:
+            assert np.abs(flow["entropy"] - other["entropy"]).max() <= 1e-10

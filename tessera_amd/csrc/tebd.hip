@@ -2279,6 +2279,8 @@ struct ExecArgs {
     int dataflow;       // per-group layout: no super-step barriers; an update waits for the
                         // done flags of its predecessors instead
     int* done;          // [updates] done flags of the dataflow order
+    unsigned long long* done_clock;  // [updates + 1] or null (dataflow): wall clock at each update's
+                                     // done flag, then at the start
     // dataflow order: snaps[0, npro) are taken before any update (their sites have
     // none earlier in the launch); snaps[usnap_begin[f], usnap_begin[f + 1]) by
     // update f after its reconstruction (it is the latest update on their site)
@@ -2441,7 +2443,10 @@ __global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs
     if (prof)
         for (int q = 0; q < kProfSlots; ++q) sh.tm[q] = 0;
     const unsigned long long t_start = prof ? wall_clock64() : 0;
-    if (a.ss_clock && prof) a.ss_clock[a.nsteps] = t_start;
+    if (blockIdx.x == 0 && tid == 0) {  // the launch's start for the completion times
+        if (a.ss_clock) a.ss_clock[a.nsteps] = wall_clock64();
+        if (a.done_clock) a.done_clock[a.up_begin[a.nsteps]] = wall_clock64();
+    }
     unsigned long long tp = t_start;
     auto mark = [&](int slot) {
         if (prof) {
@@ -2547,6 +2552,7 @@ __global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs
                     if (rank == 0 && tid == 0) {
                         __hip_atomic_store(&a.done[a.up_begin[st] + grp], 1, __ATOMIC_RELEASE,
                                            __HIP_MEMORY_SCOPE_AGENT);
+                        if (a.done_clock) a.done_clock[a.up_begin[st] + grp] = wall_clock64();
                     }
                     mark(4);
                 }
@@ -2690,12 +2696,11 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     // knob: 0 keeps the per-group layout on a full barrier after every round
     const char* pp = std::getenv("TESSERA_EXEC_P2P");
     ex_p2p_ = !(pp && pp[0] == '0');
-    // knob: the dataflow order of the per-group layout (no super-step barriers, same
-    // results): 1 always, 0 never; unset: for single-instance runs (9% faster on
-    // the n = 128 chain), while batched runs keep the super-steps (their per-instance
-    // completion times come from the super-step clocks)
+    // knob: 0 adds a grid barrier after every super-step of the per-group layout;
+    // by default its updates run in the dataflow order (same results; 9% faster on
+    // the n = 128 chain and on the batched sample)
     const char* df = std::getenv("TESSERA_EXEC_DATAFLOW");
-    ex_dataflow_ = df ? (df[0] == '1' ? 1 : 0) : -1;
+    ex_dataflow_ = df && df[0] == '0' ? 0 : 1;
     // knob: 64-cycle sleeps between the polls of a group barrier or round wait
     const char* nap = std::getenv("TESSERA_EXEC_POLL_NAPS");
     ex_naps_ = nap ? std::max(0, std::atoi(nap)) : 1;
@@ -3170,7 +3175,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     const bool group_sync = !solo && t0.ex_group_sync_;
     // dataflow order (per-group layout, no super-step barriers): every snapshot is
     // taken by the latest update on its site, or before any update when there is none
-    const bool dataflow = group_sync && (t0.ex_dataflow_ == 1 || (t0.ex_dataflow_ < 0 && ts.size() == 1));
+    const bool dataflow = group_sync && t0.ex_dataflow_ != 0;
     // XCD-local groups: grouped layout, one group per XCD (G a multiple of 8),
     // not in the dataflow order (whose done flags cross XCDs)
     bool xl = group_sync && t0.ex_xcd_local_ && !dataflow && xcd_round_robin();
@@ -3490,10 +3495,11 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.group_bar = t0.ex_bar_.p + 1;
     x.prof = t0.ex_prof_.p;
     x.ss_clock = nullptr;
-    if (!x.dataflow) {
-        if (t0.ex_clock_.n < (std::size_t)total_steps + 1) t0.ex_clock_.alloc((std::size_t)total_steps + 1);
-        x.ss_clock = t0.ex_clock_.p;
-    }
+    x.done_clock = nullptr;
+    // completion clocks: per super-step end, or in the dataflow order per update
+    const std::size_t nclk = (std::size_t)(x.dataflow ? nup : total_steps) + 1;
+    if (t0.ex_clock_.n < nclk) t0.ex_clock_.alloc(nclk);
+    (x.dataflow ? x.done_clock : x.ss_clock) = t0.ex_clock_.p;
     void* args[] = {(void*)&x};
     hipStream_t st = t0.stream_;
     // barrier counters and the error word start at zero (ordinary stream work:
@@ -3544,15 +3550,26 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     // per instance: device time from the launch start to the end of the super-step
     // holding its last update
     for (Tebd* t : ts) t->last_run_ms_ = 0.0;
-    if (x.ss_clock && total_steps > 0) {
-        std::vector<unsigned long long> clk((std::size_t)total_steps + 1);
-        TQ_HIP(hipMemcpy(clk.data(), x.ss_clock, clk.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (total_steps > 0) {
+        std::vector<unsigned long long> clk(nclk);
+        TQ_HIP(hipMemcpy(clk.data(), t0.ex_clock_.p, nclk * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         int dev = 0, khz = 0;
         TQ_HIP(hipGetDevice(&dev));
         TQ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-        for (std::size_t in = 0; in < ts.size(); ++in)
-            if (last_ss[in] >= 0 && khz > 0)
-                ts[in]->last_run_ms_ = (double)(clk[last_ss[in]] - clk[total_steps]) / khz;
+        if (x.dataflow) {  // an instance completes at the latest done flag of its updates
+            std::vector<unsigned long long> last(ts.size(), clk[nup]);
+            for (int t = 0; t < nsteps; ++t)
+                for (std::size_t k = 0; k < ups[t].size(); ++k) {
+                    unsigned long long& l = last[ups[t][k].inst];
+                    l = std::max(l, clk[up_begin[t] + k]);
+                }
+            for (std::size_t in = 0; in < ts.size(); ++in)
+                if (last_ss[in] >= 0 && khz > 0) ts[in]->last_run_ms_ = (double)(last[in] - clk[nup]) / khz;
+        } else {
+            for (std::size_t in = 0; in < ts.size(); ++in)
+                if (last_ss[in] >= 0 && khz > 0)
+                    ts[in]->last_run_ms_ = (double)(clk[last_ss[in]] - clk[total_steps]) / khz;
+        }
     }
 }
 

@@ -202,7 +202,7 @@ private:
     bool ex_prof_on_ = true;               // workgroup 0's phase profile
     bool ex_p2p_ = true;                   // per-group layout: point-to-point tournament rounds
     DevBuf<int> ex_ep_;                    // their round counters
-    int ex_dataflow_ = -1;  // per-group layout, updates wait for their predecessors only: 1 on, 0 off, -1 single instances
+    int ex_dataflow_ = 1;   // per-group layout: updates wait for their predecessors only (0: super-step barriers)
     DevBuf<int> ex_done_;                  // their done flags
     std::vector<unsigned char> ex_host_;
 

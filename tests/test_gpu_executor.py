@@ -281,7 +281,7 @@ def test_group_barrier_wait_that_gives_up_is_reported(monkeypatch):
 
 @pytest.mark.parametrize("chi,record", [(32, False), (8, False), (32, True), (8, True)])
 def test_dataflow_order_matches_superstep_barriers(monkeypatch, chi, record):
-    """In the dataflow order (the default for one instance; TESSERA_EXEC_DATAFLOW=0
+    """In the dataflow order (the default; TESSERA_EXEC_DATAFLOW=0
     keeps the super-step barriers) the per-group layout has no super-step
     barriers: each update waits only for the done flags of the latest earlier
     updates on its two sites (every dependency, bond dimensions included, goes

@@ -227,9 +227,10 @@ int tessera_tebd_batch_run_ex(tessera_tebd* const* sessions, size_t count, const
                               const double* const* b, const size_t* steps, int groups, int mode);
 /* Per-instance wall time inside the last executor launch (single or batched)
  * that advanced this session: device-clock milliseconds from the launch start
- * to the end of the super-step holding the session's last update (0 if that
- * launch gave it no update, or ran in the dataflow order). No upstream
- * counterpart: the batched scheduler's per-instance latency. */
+ * to the done flag of the session's last update (dataflow order, the default)
+ * or to the end of the super-step holding it (0 if that launch gave it no
+ * update). No upstream counterpart: the batched scheduler's per-instance
+ * latency. */
 int tessera_tebd_last_run_ms(tessera_tebd* t, double* out_ms);
 /* Lockstep executor totals since creation: super-steps run and two-site
  * updates placed in them (updates / (super-steps * groups) = slot fill). */

@@ -447,8 +447,8 @@ class TebdSession:
 
     def last_run_ms(self) -> float:
         """Device ms from the start of the last executor launch that advanced this
-        session to the end of the super-step holding its last update (the
-        per-instance wall time inside a batched run)."""
+        session to the done flag of its last update (dataflow order) or the end of
+        the super-step holding it (the per-instance wall time inside a batched run)."""
         out = ctypes.c_double()
         _check(lib().tessera_tebd_last_run_ms(self._h, ctypes.addressof(out)))
         return out.value

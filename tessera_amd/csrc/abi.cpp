@@ -21,6 +21,10 @@ const char* last_error() { return g_last_error.c_str(); }
 
 namespace {
 
+// tessera_qa_anneal(_ex): chains whose bond capacity reaches this run in the
+// lockstep executor (at n = 128, chi = 256 about 7x the per-update path)
+constexpr int kExecutorMinBond = 128;
+
 bool finite_all(const double* p, std::size_t n) {
     for (std::size_t i = 0; i < n; ++i)
         if (!std::isfinite(p[i])) return false;
@@ -91,8 +95,11 @@ int tessera_qa_anneal_ex(const double* h, size_t n, const size_t* ei, const size
         if (chi == 0) throw Error(TESSERA_ERR_ARG, "chi == 0");
         if (steps > (size_t)INT32_MAX) throw Error(TESSERA_ERR_ARG, "too many steps");
         Tebd t(make_model(h, n, ei, ej, ew, m, bias), (int)chi, dt, zz_substeps);
-        // the per-update path (one captured Trotter step replayed per step where every
-        // update's SVD takes a single-workgroup kernel)
+        // bonds that can reach 128 or more (thetas of 256 x 256 and up): the lockstep
+        // executor, every launch of up to 64 steps one cooperative kernel; smaller
+        // chains: the per-update path (one captured Trotter step replayed per step,
+        // where every update's SVD takes a single-workgroup kernel)
+        if (t.max_capacity() >= kExecutorMinBond) t.set_executor(-1, 0);
         t.run(schedule_a, schedule_b, (int)steps, 0, out_max_bond ? 0 : -1);
         t.sync();
         if (out_max_bond && steps) {

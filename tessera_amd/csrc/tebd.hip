@@ -2974,14 +2974,14 @@ void Tebd::run(const double* a, const double* b, int count, int rec0, int bond0)
     if (count <= 0) return;
     if (rec0 >= 0) ensure_trace(rec0 + count);
     if (bond0 >= 0) ensure_max_bond(bond0 + count);
-    if (exec_req_ != 0 && n_ >= 2 && bond0 < 0) {
+    if (exec_req_ != 0 && n_ >= 2 && (bond0 < 0 || rec0 >= 0)) {
         // snapshots of a recorded chunk stay resident until its entropies are
         // taken; bound the chunk to ~8 GiB of snapshots
         const std::size_t per = off_[n_] * sizeof(double2);
         const int chunk = rec0 >= 0 ? (int)std::max<std::size_t>(1, std::min<std::size_t>(64, (8ull << 30) / per)) : 64;
         for (int k0 = 0; k0 < count; k0 += chunk) {
             const int c = std::min(chunk, count - k0);
-            run_exec(a + k0, b + k0, c, rec0 >= 0 ? rec0 + k0 : -1);
+            run_exec(a + k0, b + k0, c, rec0 >= 0 ? rec0 + k0 : -1, bond0 >= 0 ? bond0 + k0 : -1);
         }
         return;
     }
@@ -3145,9 +3145,9 @@ static void exec_shape(int maxcap, int req, bool solo, bool group_sync, int work
     groups = (int)g;
 }
 
-void Tebd::run_exec(const double* a, const double* b, int count, int rec0) {
+void Tebd::run_exec(const double* a, const double* b, int count, int rec0, int bond0) {
     std::vector<Tebd*> one{this};
-    run_lockstep(one, {a}, {b}, {count}, rec0, exec_req_, 1);
+    run_lockstep(one, {a}, {b}, {count}, rec0, exec_req_, 1, bond0);
 }
 
 // The lockstep executor over one or several MPS (batched runs): every
@@ -3155,7 +3155,7 @@ void Tebd::run_exec(const double* a, const double* b, int count, int rec0) {
 // most G mutually independent updates shared by all instances.
 void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const double*>& as,
                         const std::vector<const double*>& bs, const std::vector<int>& counts, int rec0, int req,
-                        int mode) {
+                        int mode, int bond0) {
     if (ts.empty()) return;
     if (rec0 >= 0 && ts.size() != 1) throw Error(-1, "entropy recording needs a single instance");
     if (mode < 0 || mode > 2) throw Error(-1, "executor mode must be 0 (auto), 1 (grouped) or 2 (solo)");
@@ -3541,11 +3541,20 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
                            T.dims_.p, T.ex_gates_.p);
         TQ_LAUNCH_CHECK();
     }
-    // entropies of the recorded steps from their snapshots
+    // entropies of the recorded steps from their snapshots, and the largest bond
+    // after each from the snapshots' bond dimensions (every bond's last change in a
+    // step precedes both of its sites' snapshots)
     if (rec0 >= 0)
-        for (int k = 0; k < count0; ++k)
-            t0.entropy_of(t0.ex_snap_.p + (std::size_t)k * t0.off_[t0.n_],
-                          t0.ex_sdims_.p + (std::size_t)k * (t0.n_ + 1), rec0 + k);
+        for (int k = 0; k < count0; ++k) {
+            const int* sd = t0.ex_sdims_.p + (std::size_t)k * (t0.n_ + 1);
+            t0.entropy_of(t0.ex_snap_.p + (std::size_t)k * t0.off_[t0.n_], sd, rec0 + k);
+            if (bond0 >= 0) {
+                if (bond0 + k >= t0.mbond_cap_) throw Error(-1, "max-bond slot out of range");
+                hipLaunchKernelGGL(k_max_bond, dim3(1), dim3(64), 0, st, sd, t0.n_, t0.mbond_.p + bond0 + k,
+                                   nullptr);
+                TQ_LAUNCH_CHECK();
+            }
+        }
     TQ_HIP(hipStreamSynchronize(st));  // host staging buffers go out of scope
     // per instance: device time from the launch start to the end of the super-step
     // holding its last update

@@ -66,13 +66,15 @@ public:
     // entropy of step k is recorded at slot rec0 + k. Runs the steps one launch
     // sequence per update, or as one persistent launch when the executor is on.
     // bond0 >= 0: the largest bond after step k is kept at max-bond slot bond0 + k
-    // (read with max_bonds after sync; the steps then run on the per-update path).
+    // (read with max_bonds after sync; the executor takes it from the entropy
+    // snapshots, so without recording the steps run on the per-update path).
     void run(const double* a, const double* b, int count, int rec0, int bond0 = -1);
     // Lockstep executor: 0 = off, g > 0 = at most g groups of workgroups,
     // -1 = as many groups as are co-resident (capped at 64). workers: workgroups
     // per group (0 = one per block pair of a tournament round).
     void set_executor(int groups, int workers = 0);
     int executor_groups() const { return exec_req_; }
+    int max_capacity() const { return maxcap_; }  // largest bond capacity min(chi, 2^b, 2^(n-b))
     int executor_workers() const { return exec_workers_; }
     // LQ preconditioning of the multi-workgroup Jacobi SVD (on by default); in a
     // batched run the first session's setting applies.
@@ -83,9 +85,11 @@ public:
     // mode: 0 auto (solo for four or more instances with chi <= 32), 1 grouped
     // (an update's SVD spread over a group of workgroups), 2 solo (one
     // workgroup per update).
+    // bond0 >= 0 (with rec0 >= 0): the largest bond after each recorded step into
+    // max-bond slot bond0 + k, from the snapshots' bond dimensions.
     static void run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const double*>& as,
                              const std::vector<const double*>& bs, const std::vector<int>& counts, int rec0, int req,
-                             int mode = 0);
+                             int mode = 0, int bond0 = -1);
     void sync();
     std::vector<double> entropy(int count);      // recorded slots [0, count)
     double entropy_now();                        // qubit-0 entropy of the current state (mps.cpp:218-291)
@@ -126,7 +130,7 @@ public:
 
 private:
     void build_plan();
-    void run_exec(const double* a, const double* b, int count, int rec0);
+    void run_exec(const double* a, const double* b, int count, int rec0, int bond0 = -1);
     void two_site(int p, const Gate4& g, const double* wts = nullptr, int edge = -1);
     void run_graph(const double* a, const double* b, int count, int rec0, int bond0);
     void drop_graph();

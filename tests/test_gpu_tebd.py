@@ -237,6 +237,37 @@ def test_upstream_random_instances_and_readout_parity():
     assert worst < 1e-9
 
 
+def test_drop_in_entry_runs_large_bonds_in_the_executor():
+    """tessera_qa_anneal_ex runs chains whose bond capacity reaches 128 in the
+    lockstep executor (the per-step max bonds then come from the entropy
+    snapshots). n = 14 nn + nnn chain at chi = 128, exact (capacity 2^7 = 128):
+    the entropy trace equals the dense evolution of the same splitting and the
+    readout's prob its amplitude, and the max bond after step k equals the largest
+    bond that a session running the same executor over the first k steps in one
+    launch reads back at its end (the same arithmetic up to step k; a launch
+    boundary or the per-update path's SVD kernels round differently, so their
+    rank decisions at the 1e-14 cut can differ by a few)."""
+    model = I.chain_nnn(14, 61)
+    sched = I.linear(8)
+    r = qa.mps_anneal(model, sched, 0.1, 128)
+    psi, ents = dense_mps_splitting(model, sched, 0.1)
+    x = sum(1 << i for i, s_ in enumerate(r.spins) if s_ < 0)
+    prob = abs(psi[x]) ** 2 / np.vdot(psi, psi).real
+    bonds = []
+    for k in range(1, sched.steps + 1):
+        s = qa.TebdSession(model, 128, 0.1)
+        s.set_executor(-1)
+        s.run(sched.a[:k], sched.b[:k], record=True)
+        bonds.append(int(s.sync()[2][1:model.n].max()))
+        s.close()
+    print(f"max bonds {r.max_bond.tolist()}, at the end of launches of 1..{sched.steps} steps {bonds}; "
+          f"ent_err={np.abs(r.entropy - ents).max():.2e} prob={r.prob:.12f} dense={prob:.12f}", flush=True)
+    assert r.max_bond.tolist() == bonds
+    assert np.abs(r.entropy - ents).max() <= 1e-9
+    assert abs(r.prob - prob) <= 1e-10
+    assert r.max_trunc <= 1e-20
+
+
 def test_plain_abi_matches_ex_defaults():
     model = I.chain_nnn(5, 3)
     sched = I.smooth(40)

@@ -15,10 +15,11 @@ By default the timed steps (6) run in the lockstep executor: one cooperative
 launch in which consecutive Trotter steps overlap (1874 of a step's 1998 updates
 form one dependency chain, so a launch of K steps has ~K updates per super-step
 of the host's schedule while the pipeline fills; 6 steps give 5.46 on average, a
-1000-step anneal in launches of 64 steps 7.5). Each group runs its updates in that
-order and waits only for the done flags of each update's predecessors on its two
-sites (the dataflow order; TESSERA_EXEC_DATAFLOW=0 adds a grid barrier after every
-super-step). --executor 0 runs one launch sequence per update.
+1000-step anneal in launches of 64 steps 7.5). Each group takes the next update in
+that order from a shared queue and waits only for the done flags of its
+predecessors on its two sites (the dataflow order; TESSERA_EXEC_QUEUE=0 gives group
+g slot g of every super-step, TESSERA_EXEC_DATAFLOW=0 adds a grid barrier after
+every super-step). --executor 0 runs one launch sequence per update.
 
 Output: one JSON line. `value` is two-site updates per second over the timed
 steps (inputs resident on the device). `roofline` covers the dominant kernel
@@ -632,11 +633,12 @@ def tebd_timed(args, model, sched, k0, sess):
         ex1 = sess.exec_stats()
         ss = ex1["supersteps"] - ex0["supersteps"]
         kname = "k_tebd_lockstep"
-        kernel = ("k_tebd_lockstep (updates in dataflow order: theta and reconstruction on FP64 MFMA, block-Jacobi "
-                  "SVDs (FP64 VALU) on per-group barriers with point-to-point tournament rounds)")
+        kernel = ("k_tebd_lockstep (updates in dataflow order from a shared queue: theta and reconstruction on FP64 "
+                  "MFMA, block-Jacobi SVDs (FP64 VALU) on per-group barriers with point-to-point tournament rounds)")
         pr1 = sess.exec_profile()
         # the theta and reconstruction phases alone (workgroup 0's timers cover its
-        # group's updates): F of those updates / phase time, against the FP64 MFMA
+        # group's updates): the mean F per update x the updates the group ran (it takes
+        # them from the shared queue, a mix of the whole schedule) / phase time, against the FP64 MFMA
         # peak of the group's CUs (one workgroup per CU)
         nb = (2 * args.chi + 7) // 8
         per_group = args.exec_workers if args.exec_workers > 0 else max(2, nb + (nb & 1)) // 2

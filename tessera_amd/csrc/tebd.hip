@@ -2286,6 +2286,13 @@ struct ExecArgs {
     // update f after its reconstruction (it is the latest update on their site)
     const int* usnap_begin;  // [updates + 1]
     int npro;
+    // dataflow order with a shared queue: queue[0] is the next update to claim,
+    // queue[1 + g] group g's latest claim; a group claims the updates one at a time
+    // in their flat (super-step, slot) order, which is topological, so every
+    // claimed update's predecessors were claimed earlier by running groups.
+    // Null: group g runs slot g of every super-step.
+    int* queue;
+    int nup;  // updates of the launch
     unsigned long long* prof;  // [kProfSlots] phase profile of workgroup 0 (accumulated)
     unsigned long long* ss_clock;  // [nsteps + 1] or null: wall clock at the end of each super-step, then at the start
 };
@@ -2409,6 +2416,7 @@ template <int LAYOUT, bool XL = false>
 __global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs a) {
     extern __shared__ double2 lds[];
     __shared__ SvdShared sh;
+    __shared__ int s_claim;
     const int tid = threadIdx.x;
     const int G = a.G;
     const int grp = (int)(blockIdx.x % G), rank = (int)(blockIdx.x / G), nwg = (int)(gridDim.x / G);
@@ -2460,9 +2468,27 @@ __global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs
         for (int q = 0; q < a.npro; ++q) take_snapshot(a.snaps[q], I0, a.snap, a.snap_stride, a.snap_dims, a.n, tid_all, all);
         sync();
     }
-    for (int st = 0; st < a.nsteps; ++st) {
-        const bool active = grp < a.up_begin[st + 1] - a.up_begin[st];
-        const ExecUpdate& u = a.ups[active ? a.up_begin[st] + grp : 0];
+    const bool queue = LAYOUT == kExecGroupSync && a.queue != nullptr;
+    for (int st = 0; queue || st < a.nsteps; ++st) {
+        int claim = 0;
+        if constexpr (LAYOUT == kExecGroupSync) {
+            if (queue) {  // the group's next update from the shared queue
+                if (rank == 0 && tid == 0)
+                    __hip_atomic_store(&a.queue[1 + grp],
+                                       __hip_atomic_fetch_add(&a.queue[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                GroupSync<XL>{a.group_bar + 2 * grp, nwg, a.err, a.spin_limit, a.naps}();
+                if (tid == 0) s_claim = __hip_atomic_load(&a.queue[1 + grp], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __syncthreads();
+                claim = s_claim;
+                // rank 0 writes the next claim only after the update's group barriers,
+                // which every workgroup reaches after reading this one
+                if (claim >= a.nup) break;
+            }
+        }
+        const bool active = queue || grp < a.up_begin[st + 1] - a.up_begin[st];
+        const int fl = queue ? claim : (active ? a.up_begin[st] + grp : 0);  // flat update index
+        const ExecUpdate& u = a.ups[fl];
         const ExecInst& I = a.insts[active ? u.inst : 0];
         // 1. snapshots (whole grid, instance 0; the dataflow order takes them with
         // the updates), theta (group)
@@ -2542,17 +2568,15 @@ __global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs
                 if (prof) sh.tm[7] += 1;
                 if (a.dataflow) {
                     gs();  // every workgroup's part of the reconstruction is released
-                    const int f = a.up_begin[st] + grp;
-                    if (a.usnap_begin[f + 1] > a.usnap_begin[f]) {
+                    if (a.usnap_begin[fl + 1] > a.usnap_begin[fl]) {
                         // the snapshots of the sites this update leaves final for a step
-                        for (int q = a.usnap_begin[f]; q < a.usnap_begin[f + 1]; ++q)
+                        for (int q = a.usnap_begin[fl]; q < a.usnap_begin[fl + 1]; ++q)
                             take_snapshot(a.snaps[q], I0, a.snap, a.snap_stride, a.snap_dims, a.n, gid, gstride);
                         gs();  // read before a later update on the sites may rewrite them
                     }
                     if (rank == 0 && tid == 0) {
-                        __hip_atomic_store(&a.done[a.up_begin[st] + grp], 1, __ATOMIC_RELEASE,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                        if (a.done_clock) a.done_clock[a.up_begin[st] + grp] = wall_clock64();
+                        __hip_atomic_store(&a.done[fl], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                        if (a.done_clock) a.done_clock[fl] = wall_clock64();
                     }
                     mark(4);
                 }
@@ -2701,6 +2725,10 @@ Tebd::Tebd(const TebdModel& model, int chi, double dt, int zz_substeps)
     // the n = 128 chain and on the batched sample)
     const char* df = std::getenv("TESSERA_EXEC_DATAFLOW");
     ex_dataflow_ = df && df[0] == '0' ? 0 : 1;
+    // knob: 0 gives group g slot g of every super-step in the dataflow order instead of
+    // the next update from a shared queue (same results)
+    const char* qu = std::getenv("TESSERA_EXEC_QUEUE");
+    ex_queue_ = !(qu && qu[0] == '0');
     // knob: 64-cycle sleeps between the polls of a group barrier or round wait
     const char* nap = std::getenv("TESSERA_EXEC_POLL_NAPS");
     ex_naps_ = nap ? std::max(0, std::atoi(nap)) : 1;
@@ -3478,11 +3506,16 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.prof_on = t0.ex_prof_on_ ? 1 : 0;
     x.dataflow = dataflow ? 1 : 0;
     x.done = nullptr;
+    x.queue = nullptr;
+    x.nup = nup;
     if (x.dataflow) {
+        // done flags, then the queue's counter and the groups' claims
         const std::size_t nu = std::max<std::size_t>(1, up_begin[total_steps]);
-        if (t0.ex_done_.n < nu) t0.ex_done_.alloc(nu);
-        TQ_HIP(hipMemsetAsync(t0.ex_done_.p, 0, nu * sizeof(int), t0.stream_));
+        const std::size_t nq = t0.ex_queue_ ? 1 + (std::size_t)G : 0;
+        if (t0.ex_done_.n < nu + nq) t0.ex_done_.alloc(nu + nq);
+        TQ_HIP(hipMemsetAsync(t0.ex_done_.p, 0, (nu + nq) * sizeof(int), t0.stream_));
         x.done = t0.ex_done_.p;
+        if (t0.ex_queue_) x.queue = t0.ex_done_.p + nu;
     }
     if (group_sync && t0.ex_p2p_) {
         const std::size_t ne = (std::size_t)G * W * 64;

@@ -207,7 +207,8 @@ private:
     bool ex_p2p_ = true;                   // per-group layout: point-to-point tournament rounds
     DevBuf<int> ex_ep_;                    // their round counters
     int ex_dataflow_ = 1;   // per-group layout: updates wait for their predecessors only (0: super-step barriers)
-    DevBuf<int> ex_done_;                  // their done flags
+    bool ex_queue_ = true;  // dataflow order: groups take the next update from a shared queue
+    DevBuf<int> ex_done_;                  // their done flags, then the queue's counter and claims
     std::vector<unsigned char> ex_host_;
 
     // per-update path captured as a hipGraph (run_graph)

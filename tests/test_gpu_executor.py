@@ -289,7 +289,10 @@ def test_dataflow_order_matches_superstep_barriers(monkeypatch, chi, record):
     update on its site before it releases its done flag. The same rotations run
     on the same data, so the run equals the one with super-step barriers and the
     lockstep one, entropy trace included: a range-3 chain of 14 sites, 5 steps in
-    one launch, exact at chi = 32 and truncated at chi = 8."""
+    one launch, exact at chi = 32 and truncated at chi = 8. Groups take their
+    updates from a shared queue in the flat super-step order (TESSERA_EXEC_QUEUE=0
+    gives group g slot g of every super-step instead); the two runs are
+    bit-identical."""
     model = I.chain_range(14, 3, 31)
     sched = I.linear(5)
     monkeypatch.setenv("TESSERA_EXEC_DATAFLOW", "0")
@@ -298,6 +301,14 @@ def test_dataflow_order_matches_superstep_barriers(monkeypatch, chi, record):
     flow = _run(model, sched, 0.1, chi, 8, record=record)
     monkeypatch.setenv("TESSERA_EXEC_LOCKSTEP", "1")
     lock = _run(model, sched, 0.1, chi, 8, record=record)
+    monkeypatch.delenv("TESSERA_EXEC_LOCKSTEP")
+    monkeypatch.setenv("TESSERA_EXEC_QUEUE", "0")
+    slots = _run(model, sched, 0.1, chi, 8, record=record)
+    monkeypatch.delenv("TESSERA_EXEC_QUEUE")
+    # which group runs an update changes nothing in its arithmetic
+    for key in ("state", "bonds", "spins", "entropy"):
+        assert np.array_equal(flow[key], slots[key]), key
+    assert flow["max_trunc"] == slots["max_trunc"]
     for name, other in (("super-step barriers", steps), ("lockstep", lock)):
         inf = 1 - _fid(flow["state"], other["state"])
         same = np.array_equal(flow["state"], other["state"])

@@ -11,15 +11,17 @@ Workload (one "step" = one symmetric Trotter step of the MPS-TEBD anneal):
   left-isometries with bond weights ~exp(-k/40), so every interior update
   factorises a 512 x 512 complex128 theta and keeps 256 of its singular values.
 
-By default the timed steps (6) run in the lockstep executor: one cooperative
+By default the timed steps (12) run in the lockstep executor: one cooperative
 launch in which consecutive Trotter steps overlap (1874 of a step's 1998 updates
 form one dependency chain, so a launch of K steps has ~K updates per super-step
-of the host's schedule while the pipeline fills; 6 steps give 5.46 on average, a
-1000-step anneal in launches of 64 steps 7.5). Each group takes the next update in
-that order from a shared queue and waits only for the done flags of its
-predecessors on its two sites (the dataflow order; TESSERA_EXEC_QUEUE=0 gives group
-g slot g of every super-step, TESSERA_EXEC_DATAFLOW=0 adds a grid barrier after
-every super-step). --executor 0 runs one launch sequence per update.
+of the host's schedule while the pipeline fills; 6 steps give 5.46 on average, 12
+give 6.0, and the library runs a 1000-step anneal in launches of 64 steps, 7.5).
+Each group takes the next update from a shared queue, ordered by dependency depth,
+and waits only for the done flags of its predecessors on its two sites (the
+dataflow order; TESSERA_EXEC_QUEUE=0 gives group g slot g of every super-step,
+TESSERA_EXEC_DATAFLOW=0 adds a grid barrier after every super-step). --steps 6
+times the shorter launch, whose rate the pipeline's fill and drain set.
+--executor 0 runs one launch sequence per update.
 
 Output: one JSON line. `value` is two-site updates per second over the timed
 steps (inputs resident on the device). `roofline` covers the dominant kernel
@@ -55,8 +57,9 @@ REF_SWEEPS_CHI256 = 25
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6,
-                    help="timed Trotter steps (tebd: one launch; consecutive steps overlap in its super-steps)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed Trotter steps (tebd: one launch, default 12; consecutive steps overlap in its "
+                         "super-steps; the other workloads: default 6)")
     ap.add_argument("--warmup", type=int, default=0,
                     help="untimed steps (the saturated-state preparation already runs every kernel)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
@@ -88,7 +91,10 @@ def parse():
                          "-1 measured by the layout tuner (tessera_amd.tuning)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (PMC byte-counter calibration) and exit")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 12 if args.workload == "tebd" else 6
+    return args
 
 
 def progress(msg: str):

@@ -2286,12 +2286,13 @@ struct ExecArgs {
     // update f after its reconstruction (it is the latest update on their site)
     const int* usnap_begin;  // [updates + 1]
     int npro;
-    // dataflow order with a shared queue: queue[0] is the next update to claim,
-    // queue[1 + g] group g's latest claim; a group claims the updates one at a time
-    // in their flat (super-step, slot) order, which is topological, so every
-    // claimed update's predecessors were claimed earlier by running groups.
+    // dataflow order with a shared queue: queue[0] is the next claim, queue[1 + g]
+    // group g's latest claim; claim k is update qorder[k], an order in which every
+    // update comes after its predecessors, so every claimed update's predecessors
+    // were claimed earlier by running groups.
     // Null: group g runs slot g of every super-step.
     int* queue;
+    const int* qorder;  // [updates]: the flat index of the update handed out as claim k
     int nup;  // updates of the launch
     unsigned long long* prof;  // [kProfSlots] phase profile of workgroup 0 (accumulated)
     unsigned long long* ss_clock;  // [nsteps + 1] or null: wall clock at the end of each super-step, then at the start
@@ -2487,7 +2488,7 @@ __global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs
             }
         }
         const bool active = queue || grp < a.up_begin[st + 1] - a.up_begin[st];
-        const int fl = queue ? claim : (active ? a.up_begin[st] + grp : 0);  // flat update index
+        const int fl = queue ? a.qorder[claim] : (active ? a.up_begin[st] + grp : 0);  // flat update index
         const ExecUpdate& u = a.ups[fl];
         const ExecInst& I = a.insts[active ? u.inst : 0];
         // 1. snapshots (whole grid, instance 0; the dataflow order takes them with
@@ -3446,16 +3447,38 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     const std::size_t in_bytes = align(insts.size() * sizeof(ExecInst));
     const std::size_t up_bytes = align(std::max<std::size_t>(1, up_begin[total_steps]) * sizeof(ExecUpdate));
     const std::size_t sn_bytes = align(std::max<std::size_t>(1, snap_list.size()) * sizeof(ExecSnap));
-    const std::size_t int_count = (std::size_t)total_steps + 2 * (total_steps + 1) + usnap_begin.size();
-    t0.ex_host_.assign(in_bytes + up_bytes + sn_bytes + int_count * sizeof(int), 0);
-    unsigned char* hb = t0.ex_host_.data();
-    std::memcpy(hb, insts.data(), insts.size() * sizeof(ExecInst));
     for (int t = 0; t < nsteps; ++t)  // predecessors as flat update indices
         for (std::size_t k = 0; k < ups[t].size(); ++k)
             for (int q = 0; q < 2; ++q) {
                 const long long e = pred_pos[t][k][q];
                 ups[t][k].pred[q] = e < 0 ? -1 : up_begin[(int)(e >> 16)] + (int)(e & 0xffff);
             }
+    // the shared queue's order: by dependency depth (one more than the deepest
+    // predecessor's, with no cap on the updates per level), then flat order. The
+    // super-steps hold at most G updates, so once the steps in flight offer more
+    // than G independent updates they push critical updates behind later steps' ones;
+    // the uncapped depth puts them first (a CPU model of 12 steps at n = 128 with 8
+    // groups: 35.6 s against 44.6 s in the flat order; the same at 6 steps, where the
+    // critical path sets the time). Predecessors are shallower, so the order is
+    // topological and every claimed update's predecessors were claimed before it.
+    std::vector<int> qorder;
+    if (dataflow && t0.ex_queue_) {
+        std::vector<int> depth((std::size_t)nup, 0);
+        for (int t = 0; t < nsteps; ++t)
+            for (std::size_t k = 0; k < ups[t].size(); ++k) {
+                const int f = up_begin[t] + (int)k;
+                for (int q = 0; q < 2; ++q)
+                    if (ups[t][k].pred[q] >= 0) depth[f] = std::max(depth[f], depth[ups[t][k].pred[q]] + 1);
+            }
+        qorder.resize((std::size_t)nup);
+        for (int f = 0; f < nup; ++f) qorder[f] = f;
+        std::stable_sort(qorder.begin(), qorder.end(), [&](int x, int y) { return depth[x] < depth[y]; });
+    }
+    const std::size_t int_count =
+        (std::size_t)total_steps + 2 * (total_steps + 1) + usnap_begin.size() + qorder.size();
+    t0.ex_host_.assign(in_bytes + up_bytes + sn_bytes + int_count * sizeof(int), 0);
+    unsigned char* hb = t0.ex_host_.data();
+    std::memcpy(hb, insts.data(), insts.size() * sizeof(ExecInst));
     for (int t = 0; t < total_steps; ++t)
         if (!ups[t].empty())
             std::memcpy(hb + in_bytes + (std::size_t)up_begin[t] * sizeof(ExecUpdate), ups[t].data(),
@@ -3468,6 +3491,8 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     std::memcpy(hi + 2 * total_steps + 1, up_begin.data(), (total_steps + 1) * sizeof(int));
     if (!usnap_begin.empty())
         std::memcpy(hi + 3 * total_steps + 2, usnap_begin.data(), usnap_begin.size() * sizeof(int));
+    if (!qorder.empty())
+        std::memcpy(hi + 3 * total_steps + 2 + usnap_begin.size(), qorder.data(), qorder.size() * sizeof(int));
     if (t0.ex_tasks_.n < t0.ex_host_.size()) t0.ex_tasks_.alloc(t0.ex_host_.size());
     TQ_HIP(hipMemcpy(t0.ex_tasks_.p, hb, t0.ex_host_.size(), hipMemcpyHostToDevice));
 
@@ -3479,6 +3504,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.snap_begin = x.rounds + total_steps;
     x.up_begin = x.rounds + 2 * total_steps + 1;
     x.usnap_begin = usnap_begin.empty() ? nullptr : x.rounds + 3 * total_steps + 2;
+    x.qorder = qorder.empty() ? nullptr : x.rounds + 3 * total_steps + 2 + usnap_begin.size();
     x.npro = npro;
     x.small = solo ? 1 : 0;
     x.nsteps = total_steps;
@@ -3508,6 +3534,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
     x.done = nullptr;
     x.queue = nullptr;
     x.nup = nup;
+    if (x.dataflow && t0.ex_queue_ && nup > 0 && !x.qorder) throw Error(-1, "executor: the queue's order is missing");
     if (x.dataflow) {
         // done flags, then the queue's counter and the groups' claims
         const std::size_t nu = std::max<std::size_t>(1, up_begin[total_steps]);

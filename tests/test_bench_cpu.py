@@ -30,8 +30,11 @@ def test_defaults_are_the_one_gpu_headline_run(monkeypatch):
     monkeypatch.setattr(sys, "argv", ["bench.py"])
     args = bench.parse()
     assert args.gpus == 1 and args.workload == "tebd"
-    assert args.steps == 6 and args.warmup == 0
+    assert args.steps == 12 and args.warmup == 0
     assert args.n == 128 and args.chi == 256
+    # the other workloads keep their own default of 6 steps
+    monkeypatch.setattr(sys, "argv", ["bench.py", "--workload", "batch"])
+    assert bench.parse().steps == 6
 
 
 def test_host_record_of_the_cpu_baseline():

@@ -238,9 +238,10 @@ int tessera_tebd_exec_stats(tessera_tebd* t, int64_t* out_supersteps, int64_t* o
 /* Diagnostics (no upstream counterpart): phase profile of the lockstep
  * executor's first workgroup since creation, first `count` of 96 slots: ms in
  * 0 snapshots + theta, 1 LQ preconditioning, 2 Jacobi sweeps, 3 whole SVD,
- * 4 wait after the SVD, 5 reconstruction, 6 wait at the super-step end,
+ * 4 wait after the SVD, 5 reconstruction, 6 wait at the super-step end (in the
+ * dataflow order: claims from the update queue and waits for predecessors),
  * 8 waits at tournament-round barriers, 10 whole launches; counts in 7
- * super-steps with an update, 9 tournament rounds, 11 Jacobi sweeps; ms of the
+ * updates run by its group, 9 tournament rounds, 11 Jacobi sweeps; ms of the
  * rounds' parts: 12 block loads, 13 in-block pairs, 14 cross pairs,
  * 15 write-back, 16 publishing the round (point-to-point rounds); counts per
  * Jacobi sweep index s < 24 (the last index also takes later sweeps):

@@ -2446,8 +2446,9 @@ __global__ void __launch_bounds__(exec_threads(LAYOUT)) k_tebd_lockstep(ExecArgs
     }
     // phase profile of workgroup 0 (wall-clock ticks): 0 snapshots + theta,
     // 1 LQ preconditioning, 2 Jacobi sweeps, 3 whole SVD, 4 wait after the SVD,
-    // 5 reconstruction, 6 wait at the super-step end, 7 super-steps with an
-    // update, 8 waits at tournament-round barriers, 9 rounds, 10 kernel, 11 sweeps
+    // 5 reconstruction, 6 wait at the super-step end (dataflow order: claims and
+    // waits for predecessors), 7 updates run by the group, 8 waits at
+    // tournament-round barriers, 9 rounds, 10 kernel, 11 sweeps
     const bool prof = a.prof_on && blockIdx.x == 0 && tid == 0;
     if (prof)
         for (int q = 0; q < kProfSlots; ++q) sh.tm[q] = 0;
@@ -3617,7 +3618,7 @@ void Tebd::run_lockstep(const std::vector<Tebd*>& ts, const std::vector<const do
         }
     TQ_HIP(hipStreamSynchronize(st));  // host staging buffers go out of scope
     // per instance: device time from the launch start to the end of the super-step
-    // holding its last update
+    // holding its last update, or in the dataflow order to its latest done flag
     for (Tebd* t : ts) t->last_run_ms_ = 0.0;
     if (total_steps > 0) {
         std::vector<unsigned long long> clk(nclk);

@@ -99,7 +99,7 @@ public:
     double max_trunc();
     SvdStats svd_stats();
     // Executor phase profile of workgroup 0 since creation, in ms (slots 0-6,
-    // 8, 10) or counts (7 super-steps with an update, 9 rounds, 11 sweeps).
+    // 8, 10) or counts (7 updates run by its group, 9 rounds, 11 sweeps).
     std::vector<double> exec_profile();
     void set_timing(bool on);  // HIP events around every SVD launch
     std::vector<int> bonds();                    // n+1 bond dims
@@ -113,8 +113,9 @@ public:
     std::vector<double> theta(int p, int& rows, int& cols);  // identity-gate theta at (p, p+1)
     std::size_t updates_per_step() const { return plan_.size() * (std::size_t)ns_; }
     // Device time (ms) from the start of the last executor launch that advanced
-    // this session to the end of the super-step holding its last update (0 when
-    // that launch gave it no update or ran in the dataflow order).
+    // this session to the completion of its last update: the end of the super-step
+    // holding it, or in the dataflow order its latest done flag (0 when that launch
+    // gave it no update).
     double last_run_ms() const { return last_run_ms_; }
     int n() const { return n_; }
     hipStream_t stream() const { return stream_; }

@@ -648,7 +648,7 @@ def tebd_timed(args, model, sched, k0, sess):
         # peak of the group's CUs (one workgroup per CU)
         nb = (2 * args.chi + 7) // 8
         per_group = args.exec_workers if args.exec_workers > 0 else max(2, nb + (nb & 1)) // 2
-        g_updates = pr1["active_supersteps"] - pr0["active_supersteps"]
+        g_updates = pr1["group_updates"] - pr0["group_updates"]
         g_ms = (pr1["theta_ms"] - pr0["theta_ms"]) + (pr1["recon_ms"] - pr0["recon_ms"])
         g_tflops = g_updates * (f_step / max(ups_host, 1)) / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
         g_peak = FP64_PEAK_TFLOPS * per_group / 256.0

@@ -461,13 +461,13 @@ class TebdSession:
         return dict(supersteps=ss.value, updates=up.value)
 
     PROFILE_SLOTS = ("theta_ms", "lq_ms", "jacobi_ms", "svd_ms", "post_svd_wait_ms", "recon_ms",
-                     "superstep_wait_ms", "active_supersteps", "round_barrier_ms", "rounds", "kernel_ms", "sweeps",
+                     "wait_ms", "group_updates", "round_barrier_ms", "rounds", "kernel_ms", "sweeps",
                      "round_load_ms", "round_inblock_ms", "round_cross_ms", "round_writeback_ms",
                      "round_publish_ms")
 
     def exec_profile(self) -> dict:
         """Phase profile of the lockstep executor's first workgroup since creation
-        (ms, or counts for active_supersteps, rounds and sweeps)."""
+        (ms, or counts for group_updates (updates run by its group), rounds and sweeps)."""
         out = np.zeros(len(self.PROFILE_SLOTS))
         _check(lib().tessera_tebd_exec_profile(self._h, _p(out), out.size))
         return dict(zip(self.PROFILE_SLOTS, out.tolist()))

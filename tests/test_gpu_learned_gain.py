@@ -7,7 +7,7 @@ fixed-seed held-out instances -- including one at n = 128, chi = 128 -- three
 anneals of the same shortened length (half the plain length; 100 plain steps,
 40 for the n = 128 instance) are compared with
 the best plain energy of each instance (the better of the full-length and the
-half-length plain anneal; plain TEBD is the judge):
+half-length plain anneal; plain TEBD is the yardstick):
 
 * plain TEBD at half length,
 * guided by the trained network (warm-start bias b = z_pred, the same length),
